@@ -1,0 +1,228 @@
+/*
+ * pdhg_hip.h - C-ABI of the MI355X-native restarted-PDHG engine (gfx950).
+ *
+ * This is the drop-in boundary for the reference solver's PDHG engine
+ * (src/pdhg/pdhg.h, src/pdhg/pdhg.cpp and src/gpu/cuda_driver.h of the
+ * reference).  Plain pointers and sizes only; no exception crosses it.
+ * Functions returning int give 0 on success and -1 on error, with the message
+ * in pdhg_last_error() (thread-local), as the reference C API does with
+ * pramana_last_error (include/pramana/pramana.h:44, src/api/c_api.cpp:19).
+ * Functions returning a handle give NULL on error.
+ *
+ * Problem form (src/core/model.h:1-10): min c'x + offset  s.t.
+ * row_lower <= A x <= row_upper, col_lower <= x <= col_upper, A in CSC.
+ * A bound with |v| >= 1e20 is infinite (src/util/common.h:15).
+ *
+ * Threading: one caller per handle at a time; engine entries may be called
+ * from any thread.  All GPU work runs on the engine's own HIP stream.
+ */
+#ifndef PDHG_HIP_H
+#define PDHG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status values keep the numbering of the reference Status enum
+ * (src/util/common.h:34-46). */
+enum {
+  PDHG_NOT_SOLVED = 0,
+  PDHG_OPTIMAL = 1,
+  PDHG_TIME_LIMIT = 5,
+  PDHG_ITERATION_LIMIT = 6,
+  PDHG_NUMERICAL_FAILURE = 8,
+  PDHG_ERROR = 10
+};
+
+/* Backend vector ids, same order as the reference `enum V`
+ * (src/pdhg/pdhg.cpp:45).  Column vectors (X..UP) have n*k entries, row
+ * vectors (Y..RU) m*k, both k-interleaved: element (i, b) at i*k + b. */
+enum {
+  PDHG_V_X = 0, PDHG_V_XP, PDHG_V_X0, PDHG_V_ATY, PDHG_V_ATYP, PDHG_V_C, PDHG_V_LO, PDHG_V_UP,
+  PDHG_V_Y, PDHG_V_YP, PDHG_V_Y0, PDHG_V_AX, PDHG_V_AXP, PDHG_V_AX0, PDHG_V_RL, PDHG_V_RU,
+  PDHG_NV
+};
+
+/* Iterate precision. */
+enum { PDHG_FP32 = 0, /* FP32 iterates + SpMV, FP64 norms/restarts/KKT */
+       PDHG_FP64 = 1  /* FP64 everywhere (the reference's arithmetic)   */ };
+
+/* Read-only LP description (minimization form), CSC matrix. */
+typedef struct pdhg_lp {
+  int m, n;
+  const int* col_start;    /* n+1 */
+  const int* row_index;    /* nnz */
+  const double* value;     /* nnz */
+  const double* cost;      /* n */
+  const double* col_lower; /* n */
+  const double* col_upper; /* n */
+  const double* row_lower; /* m */
+  const double* row_upper; /* m */
+  double obj_offset;
+} pdhg_lp;
+
+/* One case of a family sharing A: LpCase (src/pdhg/pdhg.h:68-70).  NULL
+ * means "same as the base model".  x0 / y0 (original space) are a warm
+ * start, which the reference does not have (it hard-wires x0 = clamp(0),
+ * y0 = 0, src/pdhg/pdhg.cpp:509,519); NULL keeps that cold start. */
+typedef struct pdhg_case {
+  const double* cost;
+  const double* col_lower;
+  const double* col_upper;
+  const double* row_lower;
+  const double* row_upper;
+  const double* x0;
+  const double* y0;
+} pdhg_case;
+
+/* PdhgOptions (src/pdhg/pdhg.h:32-42) plus the engine's own fields. */
+typedef struct pdhg_options {
+  double tolerance;            /* relative KKT tolerance (1e-6) */
+  long long max_iterations;    /* 1000000 */
+  int check_every;             /* iterations between checks (64); >= 2 */
+  int use_gpu;                 /* must be 1: there is no CPU fallback */
+  int ruiz_iterations;         /* 10 */
+  int pock_chambolle;          /* 1 */
+  int rescale;                 /* objective/bound rescaling beta, gamma (1) */
+  double reflection;           /* rho (1.0) */
+  int log_interval;            /* log every N checks to stderr (0 = off) */
+  int precision;               /* PDHG_FP32 (default) or PDHG_FP64 */
+  int use_graph;               /* capture check_every iterations in a hipGraph (1) */
+  int lanes_per_row;           /* 0 = autotune; else nnz-per-lane target (1..64) */
+  int row_map;                 /* 0 = auto, 1 = row-length buckets, 2 = one mapping */
+  int xcd_remap;               /* -1 = auto, 0 = off, 1 = contiguous rows per XCD */
+  double time_limit;           /* seconds, <= 0: none (Deadline, src/util/timer.h:22) */
+  int device;                  /* HIP device ordinal (0) */
+} pdhg_options;
+
+/* PdhgStats (src/pdhg/pdhg.h:44-56) plus engine counters. */
+typedef struct pdhg_stats {
+  long long iterations;
+  int restarts;
+  double primal_residual, dual_residual, gap, primal_weight;
+  double setup_seconds, solve_seconds, kernel_seconds, transfer_seconds;
+  long long bytes_uploaded, bytes_downloaded, kernel_launches, graph_launches;
+  double spectral_norm;
+  int batch;                   /* instances in the run */
+  int precision;
+  double spmv_bytes_per_iter;  /* algorithmic bytes of the 2 SpMVs per iteration */
+  double iter_bytes;           /* algorithmic bytes of one fused iteration */
+  /* Optional per-check trace of instance 0 (caller-owned, may be NULL). */
+  double* trace;               /* records of 8 doubles: it, relP, relD, gap, pobj, dobj, omega, restarted */
+  int trace_capacity;
+  int trace_length;
+  char backend[32];
+  char device_name[64];
+} pdhg_stats;
+
+/* PdhgResult (src/pdhg/pdhg.h:58-65).  x (n) and row_dual (m) are
+ * caller-allocated and may be NULL. */
+typedef struct pdhg_result {
+  int status;
+  int restarts;
+  long long iterations;
+  double objective, dual_objective, relative_kkt;
+  double rel_primal, rel_dual, rel_gap, primal_weight;
+  double* x;
+  double* row_dual;
+} pdhg_result;
+
+/* ---- errors / device --------------------------------------------------- */
+const char* pdhg_last_error(void);
+void pdhg_default_options(pdhg_options* opts);
+/* gpuAvailable (src/pdhg/pdhg.h:79). */
+int pdhg_gpu_available(void);
+/* gpuInfoJson (src/pdhg/pdhg.h:80, src/pdhg/pdhg.cpp:732-746): writes a JSON
+ * object with name, gfx arch, CUs, memory, peak bandwidth into buf. */
+int pdhg_gpu_info_json(char* buf, size_t cap);
+
+/* ---- engine entries ---------------------------------------------------- */
+/* solvePdhg (src/pdhg/pdhg.h:73).  stats may be NULL. */
+int pdhg_solve(const pdhg_lp* lp, const pdhg_options* opts, pdhg_result* result, pdhg_stats* stats);
+/* solvePdhgBatch (src/pdhg/pdhg.h:76-77): k cases sharing A in one batched
+ * run; results[k].  Beta/gamma/scaling come from the base model as in the
+ * reference (src/pdhg/pdhg.cpp:441-450). */
+int pdhg_solve_batch(const pdhg_lp* base, const pdhg_case* cases, int k, const pdhg_options* opts,
+                     pdhg_result* results, pdhg_stats* batch_stats);
+/* Packed uneven batch: `count` unrelated LPs solved together by one
+ * persistent kernel with per-instance restart state and a device work
+ * queue.  No reference counterpart (the reference solves such LPs one by
+ * one through solvePdhg, src/pdhg/pdhg.h:73). */
+int pdhg_solve_packed(const pdhg_lp* lps, int count, const pdhg_options* opts, pdhg_result* results,
+                      pdhg_stats* stats);
+
+/* ---- op-level backend (the Backend vtable, src/pdhg/pdhg.cpp:51-70) ----- */
+typedef struct pdhg_backend pdhg_backend;
+/* init(A, AT, rfac, cfac, k): CSR of the scaled A (m rows) and of its
+ * transpose (n rows), both FP64 on the host. */
+pdhg_backend* pdhg_backend_create(int m, int n, const int* a_ptr, const int* a_idx, const double* a_val,
+                                  const int* t_ptr, const int* t_idx, const double* t_val, const double* rfac,
+                                  const double* cfac, int k, int precision, int device);
+void pdhg_backend_destroy(pdhg_backend* be);
+int pdhg_backend_set(pdhg_backend* be, int vec, const double* data);   /* set (pdhg.cpp:56) */
+int pdhg_backend_get(pdhg_backend* be, int vec, double* out);          /* get (pdhg.cpp:57) */
+int pdhg_backend_spmv(pdhg_backend* be, int in, int out);              /* spmv (pdhg.cpp:58) */
+int pdhg_backend_spmvT(pdhg_backend* be, int in, int out);             /* spmvT (pdhg.cpp:59) */
+int pdhg_backend_set_steps(pdhg_backend* be, const double* tau, const double* sigma); /* pdhg.cpp:60 */
+int pdhg_backend_primal_step(pdhg_backend* be);                        /* pdhg.cpp:61 */
+int pdhg_backend_dual_step(pdhg_backend* be);                          /* pdhg.cpp:62 */
+int pdhg_backend_halpern(pdhg_backend* be, double rho);                /* pdhg.cpp:63 */
+int pdhg_backend_restart(pdhg_backend* be, const int* mask);           /* pdhg.cpp:64 */
+int pdhg_backend_kkt(pdhg_backend* be, double* rows3k, double* cols3k); /* pdhg.cpp:65 */
+int pdhg_backend_diff2(pdhg_backend* be, int a, int b, double* out);   /* pdhg.cpp:66 */
+/* Times `reps` back-to-back SpMVs (transpose: 0 = A x, 1 = A^T y) with HIP
+ * events on the backend stream; writes the mean kernel time in ms. */
+int pdhg_backend_time_spmv(pdhg_backend* be, int transpose, int reps, double* ms);
+
+/* ---- certifier handoff (src/cert/certifier.h:198-201) ------------------ */
+/* Rigorous Neumaier-Shcherbina lower bound on min c'x valid for any y,
+ * outward-rounded FP64 interval arithmetic on the host. */
+int pdhg_safe_dual_bound(const pdhg_lp* lp, const double* y, double* bound, int* infinite_terms,
+                         double* cost_perturbation);
+/* Family acceptance of the reference (src/parametric/parametric.cpp:509-524):
+ * implied bounds by activity propagation, safe bound, |obj - LB| <=
+ * 1e-4 (1 + |obj|).  out[4] = {safe bound, |obj-LB|/(1+|obj|), max primal
+ * violation (relative), accepted (0/1)}. */
+int pdhg_certify(const pdhg_lp* lp, const double* x, const double* y, double objective, double* out);
+/* The same acceptance for k cases of a family (the per-case loop of
+ * src/parametric/parametric.cpp:505-525) on `threads` host threads; cases
+ * that share bounds share one propagation.  xs (k x n, may be NULL), ys
+ * (k x m), objectives[k]; out[4k] as pdhg_certify. */
+int pdhg_certify_batch(const pdhg_lp* base, const pdhg_case* cases, int k, const double* xs, const double* ys,
+                       const double* objectives, int threads, double* out);
+
+/* ---- synthetic LP generators (owned models) ---------------------------- */
+typedef struct pdhg_model pdhg_model;
+/* Random LP with a known optimum: x* half zeros / half U(0,1), s*
+ * complementary U(0,1), y* ~ N(0,1), b = A x* (equality rows), c = A'y* + s*,
+ * x >= 0.  row_sigma == 0: every column gets nnz_per_col entries at uniform
+ * rows (duplicates merged).  row_sigma > 0: row lengths are log-normal(sigma)
+ * scaled to mean nnz_per_col * n / m, columns uniform within each row. */
+pdhg_model* pdhg_gen_random_lp(uint64_t seed, int m, int n, double nnz_per_col, double row_sigma);
+/* Refinery-style multi-period blending LP (block staircase, inventory
+ * linking rows), every column bounded, rows and columns scaled by
+ * 10^U(-scale_decades, scale_decades). */
+pdhg_model* pdhg_gen_refinery(uint64_t seed, int periods, int streams, int crudes, int products, int qualities,
+                              double scale_decades);
+/* Price family of a refinery model: cost_k = c0 + sum_q theta_kq d_q, where
+ * d_q is crude q's base purchase price on its purchase columns (q < 4) and
+ * theta_kq ~ U[-range, range].  thetas (k x 4) and costs (k x n) are written. */
+int pdhg_gen_family(const pdhg_model* base, uint64_t seed, int k, double range, double* thetas, double* costs);
+/* Same family map for given thetas (k x 4). */
+int pdhg_family_costs(const pdhg_model* base, int k, const double* thetas, double* costs);
+/* The reference's `calibrate` LP (src/parametric/parametric.cpp:543-563):
+ * m = s rows >= 1, n = 2s columns in [0, 10], 4 entries 0.5 + U per column,
+ * cost U - 0.3.  Used to fit the CPU/GPU crossover. */
+pdhg_model* pdhg_gen_calibration(uint64_t seed, int s);
+/* Borrowed view of a model; x_star / y_star may be NULL (refinery). */
+int pdhg_model_view(const pdhg_model* mdl, pdhg_lp* lp, const double** x_star, const double** y_star);
+void pdhg_model_free(pdhg_model* mdl);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDHG_HIP_H */

@@ -1,0 +1,84 @@
+// Interface between the host engine (plain C++) and the HIP side.  Nothing
+// here names a HIP type, so the host translation units build with g++.
+#pragma once
+
+#include <memory>
+#include <string>
+
+#include "prepare.h"
+
+namespace pdhgamd {
+
+struct DeviceInfo {
+  bool available = false;
+  std::string reason, name, arch;
+  int computeUnits = 0;
+  size_t memoryBytes = 0;
+  double peakBandwidthGBs = 0;
+  int ordinal = 0;
+};
+
+// Probes the HIP runtime once per process (the analogue of the reference's
+// CudaContext singleton, src/gpu/cuda_driver.cpp:148-162).
+const DeviceInfo& device_info(int device);
+
+// Row-to-lane mapping of one CSR matrix.
+struct MapSetup {
+  int nnzPerLane = 4;  // target nonzeros per lane: lanes per row = nextpow2(ceil(len / nnzPerLane))
+  int rowMap = 1;      // 1: row-length buckets (rows permuted); 2: one mapping for all rows
+  bool xcd = false;    // give each XCD a contiguous range of rows
+};
+
+struct RunSetup {
+  int m = 0, n = 0;
+  int B = 1;           // instances per run: a power of two <= 64
+  int precision = 0;   // 0: FP32 iterates, FP64 norms; 1: FP64
+  bool sharedCost = false, sharedColBounds = false, sharedRowBounds = false;
+  MapSetup colMap, rowMap;
+  double rho = 1.0;
+  int checkEvery = 64;
+  bool useGraph = true;
+};
+
+// Per-instance record written at every check (8 doubles, the trace format).
+struct CheckRecord {
+  double it, relP, relD, gap, pobj, dobj, omega, restarted;
+};
+
+struct RunScalars {
+  double eta = 1, beta = 1, gamma = 1, offset = 0, tol = 1e-6;
+};
+
+// Device state of one batched restarted-PDHG run: B instances sharing A.
+class DeviceRun {
+ public:
+  virtual ~DeviceRun() = default;
+  static std::unique_ptr<DeviceRun> create(int device, const Prepared& P, const RunSetup& s);
+
+  // Scaled case data, FP64 on the host: k-interleaved (i*B + b) or, for the
+  // arrays flagged shared in RunSetup, one value per row / column.
+  virtual void upload(const double* c, const double* lo, const double* up, const double* rl, const double* ru,
+                      const double* x, const double* y) = 0;
+  // Per-instance scalars; done[b] != 0 marks a padding lane.
+  virtual void init_state(const RunScalars& s, const double* omega, const double* cNorm, const double* rhsNorm,
+                          const int* done) = 0;
+  // ax = A x, ax0 = ax; starts the loop timer.
+  virtual void start() = 0;
+  // `iters` fused iterations, the last one a check (KKT, restart, primal weight).
+  virtual void run_block(int iters) = 0;
+  // Waits for the stream, then copies the last check records and done flags.
+  virtual void fetch(CheckRecord* rec, int* done, long long* doneIter, int* restarts) = 0;
+  // k-interleaved copy of an iterate (0: x+, 1: y+) in FP64.
+  virtual void download(int which, double* out) = 0;
+  // Stops the loop timer: device seconds since start().
+  virtual double stop() = 0;
+  // Mean duration (ms) of the column and row iteration kernels, timed with
+  // HIP events on the run's stream over `reps` launches each.
+  virtual void time_kernels(int reps, double* msCol, double* msRow) = 0;
+  // Algorithmic bytes of one launch of the column and row iteration kernels.
+  virtual void kernel_bytes(double* colBytes, double* rowBytes) const = 0;
+
+  long long launches = 0, graphLaunches = 0, bytesUp = 0, bytesDown = 0;
+};
+
+}  // namespace pdhgamd

@@ -1,0 +1,26 @@
+// Host engine: the reference's solvePdhg / solvePdhgBatch
+// (src/pdhg/pdhg.h:73-77) over the HIP device run.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/pdhg_hip.h"
+#include "core.h"
+
+namespace pdhgamd {
+
+// One case of a family (LpCase, src/pdhg/pdhg.h:68-70); NULL = base model.
+struct CaseData {
+  const double *c = nullptr, *l = nullptr, *u = nullptr, *L = nullptr, *U = nullptr;
+  const double *x0 = nullptr, *y0 = nullptr;  // warm start (original space)
+};
+
+// Runs all cases (any count; chunks of at most 64 share one set-up) and fills
+// results[k] and stats.
+void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pdhg_options& o, pdhg_result* results,
+                 pdhg_stats* stats);
+
+Model model_from(const pdhg_lp* lp);
+
+}  // namespace pdhgamd

@@ -1,0 +1,1362 @@
+// HIP kernels of the restarted-PDHG engine for gfx950 (MI355X, CDNA4) and the
+// device runtime that drives them.
+//
+// Layout: B instances share the scaled matrix; an iterate element (i, b) sits
+// at i*B + b, so for every nonzero a wavefront reads B consecutive values.
+// A row is served by a group of B*S lanes (lane = instance + B * sub-lane); S
+// (a power of two) is chosen per row-length bucket so that each lane streams a
+// few consecutive nonzeros.  Rows of one bucket are packed into 256-thread
+// workgroups; a workgroup only ever holds rows of one bucket, so shuffle
+// reductions are uniform inside it.  No atomics anywhere: every cross-lane or
+// cross-block sum is a fixed-order tree.
+//
+// One PDHG iteration is two kernels (reference src/pdhg/pdhg.cpp:588-662):
+//   k_col : aty = A^T y, x+ = proj(x - tau (c - aty)), Halpern update of x
+//   k_row : ax+ = A x+, reflected dual step, Halpern update of y and ax
+// and a check iteration adds k_colcheck (FP64 A^T y+ and the column KKT terms)
+// and k_finalize (FP64 KKT / gap / restart / primal weight, one block per
+// instance).  Restarts are applied by the first iteration after the check.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pdhg_hip.h"
+#include "device.h"
+
+namespace pdhgamd {
+
+#define HIP_OK(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t _e = (expr);                                                                            \
+    if (_e != hipSuccess)                                                                              \
+      throw ::pdhgamd::Error(std::string("HIP error '") + hipGetErrorString(_e) + "' at " #expr);      \
+  } while (0)
+
+namespace {
+
+constexpr int kWG = 256;       // threads per workgroup (4 wave64)
+constexpr int kLogWG = 8;
+constexpr int kMaxBuckets = 8;
+
+// ---------------------------------------------------------------------------
+// Row mapping
+// ---------------------------------------------------------------------------
+struct RowPlan {
+  int nb;                          // buckets in use
+  int blkStart[kMaxBuckets + 1];   // first workgroup of each bucket
+  int rowStart[kMaxBuckets];       // offset of the bucket in perm (or its first row)
+  int rowCount[kMaxBuckets];
+  int logS[kMaxBuckets];           // log2 lanes per row (per instance)
+  const int* perm;                 // row order, NULL = identity
+  int xcd;                         // 1: contiguous row ranges per XCD
+};
+
+// Workgroup ids are dispatched round-robin over the 8 XCDs; this bijection
+// gives XCD x a contiguous range of logical blocks, so neighbouring rows (which
+// share columns) meet in the same L2.
+__device__ inline int remap_xcd(int bid, int nblk) {
+  const int q = nblk >> 3, r = nblk & 7;
+  const int x = bid & 7, l = bid >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
+}
+
+struct Slot {
+  int row;   // -1: no row for this lane
+  int sub;   // sub-lane within the row group
+  int logS;
+};
+
+template <int LOGB>
+__device__ inline Slot locate(const RowPlan& p, int bid) {
+  int q = 0;
+  while (q + 1 < p.nb && bid >= p.blkStart[q + 1]) ++q;
+  Slot s;
+  s.logS = p.logS[q];
+  const int logG = LOGB + s.logS;
+  const int t = threadIdx.x;
+  const int r = ((bid - p.blkStart[q]) << (kLogWG - logG)) + (t >> logG);
+  s.sub = (t & ((1 << logG) - 1)) >> LOGB;
+  s.row = r < p.rowCount[q] ? (p.perm ? p.perm[p.rowStart[q] + r] : p.rowStart[q] + r) : -1;
+  return s;
+}
+
+__device__ inline int block_id(const RowPlan& p) { return p.xcd ? remap_xcd(blockIdx.x, gridDim.x) : blockIdx.x; }
+
+// Sum over the S sub-lanes of a row group (lanes b, b+B, b+2B, ...).
+template <int LOGB, typename T>
+__device__ inline T reduce_sub(T v, int logS) {
+  for (int o = 1 << LOGB; o < (1 << (LOGB + logS)); o <<= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// sum_e val[e] * x[idx[e]*B + b] over e = e0, e0+step, ... < e1, with four
+// independent chains so the index, value and gather loads of four nonzeros are
+// in flight together.
+template <int LOGB, typename Acc, typename V, typename X>
+__device__ inline Acc row_dot(const int* __restrict__ idx, const V* __restrict__ val, const X* __restrict__ x,
+                              int e0, int e1, int step, int b) {
+  Acc a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int e = e0;
+  for (; e + 3 * step < e1; e += 4 * step) {
+    const int c0 = idx[e], c1 = idx[e + step], c2 = idx[e + 2 * step], c3 = idx[e + 3 * step];
+    const Acc v0 = val[e], v1 = val[e + step], v2 = val[e + 2 * step], v3 = val[e + 3 * step];
+    a0 += v0 * static_cast<Acc>(x[(static_cast<size_t>(c0) << LOGB) + b]);
+    a1 += v1 * static_cast<Acc>(x[(static_cast<size_t>(c1) << LOGB) + b]);
+    a2 += v2 * static_cast<Acc>(x[(static_cast<size_t>(c2) << LOGB) + b]);
+    a3 += v3 * static_cast<Acc>(x[(static_cast<size_t>(c3) << LOGB) + b]);
+  }
+  for (; e < e1; e += step) a0 += static_cast<Acc>(val[e]) * static_cast<Acc>(x[(static_cast<size_t>(idx[e]) << LOGB) + b]);
+  return (a0 + a1) + (a2 + a3);
+}
+
+// Per-workgroup FP64 partial sums of NQ quantities for each instance
+// b = threadIdx % B, stored at out[((q0 + q) * B + b) * nblk + blk].
+template <int LOGB, int NQ>
+__device__ inline void block_partials(const double (&v)[NQ], double* __restrict__ out, int q0, int nblk, int blk) {
+  constexpr int B = 1 << LOGB;
+  __shared__ double sh[NQ][kWG / 64][B];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double s = v[q];
+    for (int o = 32; o >= B; o >>= 1) s += __shfl_xor(s, o);
+    if (lane < B) sh[q][wave][lane] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < B) {
+    const int b = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      out[(static_cast<size_t>(q0 + q) * B + b) * nblk + blk] = ((sh[q][0][b] + sh[q][1][b]) + sh[q][2][b]) + sh[q][3][b];
+  }
+}
+
+template <typename T>
+__device__ inline T clamp_to(T v, T lo, T hi) {
+  v = v > lo ? v : lo;  // max(v, lo) then min(., hi): src/pdhg/pdhg.cpp:121-122
+  return v < hi ? v : hi;
+}
+
+// ---------------------------------------------------------------------------
+// Per-instance state (device arrays of length B)
+// ---------------------------------------------------------------------------
+struct Inst {
+  double *tau, *sigma, *omega, *fpR, *fpPrev;
+  long long *iter, *since, *restartAt, *applyAt, *doneIter;
+  int *done, *restarts;
+  const double *cNorm, *rhsNorm;
+  CheckRecord* rec;
+};
+
+// Stride of a per-instance (ld = B, mask = B-1) or shared (ld = 1, mask = 0) array.
+struct Ld {
+  int ld, mask;
+  __device__ size_t at(int i, int b) const { return static_cast<size_t>(i) * ld + (b & mask); }
+};
+
+// ---------------------------------------------------------------------------
+// Iteration kernels
+// ---------------------------------------------------------------------------
+
+// Column side of an iteration over the rows of A^T (reference pdhg.cpp:589-590,
+// 116-125, 138-152): x+ = clamp(x - tau (c - A^T y), lo, up), then
+// x <- lam((1+rho) x+ - rho x) + (1-lam) x0 with lam = (k+1)/(k+2).
+// CHECK adds ||x - x+||^2 and ||x+ - x0||^2 (quantities 0, 1).
+template <typename T, int LOGB, bool CHECK>
+__global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                             const T* __restrict__ val, const T* __restrict__ y, T* __restrict__ x,
+                                             T* __restrict__ xp, T* __restrict__ x0, const T* __restrict__ c, Ld lc,
+                                             const T* __restrict__ lo, const T* __restrict__ up, Ld lb, Inst st,
+                                             int tloc, double rho, double* __restrict__ part, int nblk) {
+  constexpr int B = 1 << LOGB;
+  const int bid = block_id(plan);
+  const Slot s = locate<LOGB>(plan, bid);
+  const int b = threadIdx.x & (B - 1);
+  T acc = 0;
+  if (s.row >= 0) acc = row_dot<LOGB, T>(idx, val, y, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
+  acc = reduce_sub<LOGB>(acc, s.logS);
+  double q[2] = {0.0, 0.0};
+  if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
+    const int j = s.row;
+    const size_t t = (static_cast<size_t>(j) << LOGB) + b;
+    const long long it = st.iter[b] + tloc;
+    const bool rst = st.applyAt[b] == it;
+    const T xin = rst ? xp[t] : x[t];
+    T xo;
+    if (rst) {
+      xo = xin;
+      x0[t] = xin;
+    } else {
+      xo = x0[t];
+    }
+    const T tau = static_cast<T>(st.tau[b]);
+    const T v = clamp_to<T>(xin - tau * (c[lc.at(j, b)] - acc), lo[lb.at(j, b)], up[lb.at(j, b)]);
+    const double kk = static_cast<double>(it - st.restartAt[b]);
+    const double lam = (kk + 1.0) / (kk + 2.0);
+    x[t] = static_cast<T>(lam) * (static_cast<T>(1.0 + rho) * v - static_cast<T>(rho) * xin) +
+           static_cast<T>(1.0 - lam) * xo;
+    xp[t] = v;
+    if (CHECK) {
+      const double d = static_cast<double>(xin) - static_cast<double>(v);
+      const double e = static_cast<double>(v) - static_cast<double>(xo);
+      q[0] = d * d;
+      q[1] = e * e;
+    }
+  }
+  if (CHECK) block_partials<LOGB, 2>(q, part, 0, nblk, bid);
+}
+
+// Row side of an iteration over the rows of A (reference pdhg.cpp:591-592,
+// 126-137, 138-152): ax+ = A x+, v = y - sigma (2 ax+ - ax),
+// y+ = v + sigma clamp(-v / sigma, L, U), Halpern updates of y and ax.
+// CHECK also stores y+ and ax+, recomputes A x+ with FP64 values and FP64
+// accumulation, and adds (quantities 0..3): sum res^2, the row part of the
+// dual objective, ||y - y+||^2, ||y+ - y0||^2.
+template <typename T, int LOGB, bool CHECK>
+__global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                             const T* __restrict__ val, const double* __restrict__ val64,
+                                             const T* __restrict__ xp, T* __restrict__ y, T* __restrict__ yp,
+                                             T* __restrict__ y0, T* __restrict__ ax, T* __restrict__ axp,
+                                             T* __restrict__ ax0, const T* __restrict__ rl, const T* __restrict__ ru,
+                                             const double* __restrict__ rl64, const double* __restrict__ ru64, Ld lr,
+                                             const double* __restrict__ rfac, Inst st, int tloc, double rho,
+                                             double* __restrict__ part, int nblk) {
+  constexpr int B = 1 << LOGB;
+  const int bid = block_id(plan);
+  const Slot s = locate<LOGB>(plan, bid);
+  const int b = threadIdx.x & (B - 1);
+  T acc = 0;
+  double acc64 = 0;
+  if (s.row >= 0) {
+    const int e0 = ptr[s.row] + s.sub, e1 = ptr[s.row + 1], step = 1 << s.logS;
+    acc = row_dot<LOGB, T>(idx, val, xp, e0, e1, step, b);
+    if (CHECK) acc64 = row_dot<LOGB, double>(idx, val64, xp, e0, e1, step, b);
+  }
+  acc = reduce_sub<LOGB>(acc, s.logS);
+  if (CHECK) acc64 = reduce_sub<LOGB>(acc64, s.logS);
+  double q[4] = {0.0, 0.0, 0.0, 0.0};
+  if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
+    const int i = s.row;
+    const size_t t = (static_cast<size_t>(i) << LOGB) + b;
+    const long long it = st.iter[b] + tloc;
+    const bool rst = st.applyAt[b] == it;
+    const T yin = rst ? yp[t] : y[t];
+    const T axin = rst ? axp[t] : ax[t];
+    T yo, axo;
+    if (rst) {
+      yo = yin;
+      axo = axin;
+      y0[t] = yin;
+      ax0[t] = axin;
+    } else {
+      yo = y0[t];
+      axo = ax0[t];
+    }
+    const T sg = static_cast<T>(st.sigma[b]);
+    const T v = yin - sg * (static_cast<T>(2) * acc - axin);
+    const T w = clamp_to<T>(-v / sg, rl[lr.at(i, b)], ru[lr.at(i, b)]);
+    const T yplus = v + sg * w;
+    const double kk = static_cast<double>(it - st.restartAt[b]);
+    const double lam = (kk + 1.0) / (kk + 2.0);
+    const T lamT = static_cast<T>(lam), omT = static_cast<T>(1.0 - lam);
+    const T r1 = static_cast<T>(1.0 + rho), r0 = static_cast<T>(rho);
+    y[t] = lamT * (r1 * yplus - r0 * yin) + omT * yo;
+    ax[t] = lamT * (r1 * acc - r0 * axin) + omT * axo;
+    if (CHECK) {
+      yp[t] = yplus;
+      axp[t] = acc;
+      const double L = rl64[lr.at(i, b)], U = ru64[lr.at(i, b)];
+      const double res = (fmax(L - acc64, 0.0) + fmax(acc64 - U, 0.0)) * rfac[i];
+      const double yy = static_cast<double>(yplus);
+      q[0] = res * res;
+      if (yy > 0 && L > -1e300) q[1] = yy * L;
+      else if (yy < 0 && U < 1e300) q[1] = yy * U;
+      const double d = static_cast<double>(yin) - yy, e = yy - static_cast<double>(yo);
+      q[2] = d * d;
+      q[3] = e * e;
+    }
+  }
+  if (CHECK) block_partials<LOGB, 4>(q, part, 0, nblk, bid);
+}
+
+// Column KKT terms at (x+, y+) in FP64 (reference pdhg.cpp:181-196):
+// d = c - A^T y+ ; dual residual (by bound pattern) * cfac, c'x+, and the
+// column part of the dual objective (quantities 2, 3, 4).
+template <typename T, int LOGB>
+__global__ __launch_bounds__(kWG) void k_colcheck(RowPlan plan, const int* __restrict__ ptr,
+                                                  const int* __restrict__ idx, const double* __restrict__ val64,
+                                                  const T* __restrict__ yp, const T* __restrict__ xp,
+                                                  const double* __restrict__ c64, Ld lc,
+                                                  const double* __restrict__ lo64, const double* __restrict__ up64,
+                                                  Ld lb, const double* __restrict__ cfac, Inst st,
+                                                  double* __restrict__ part, int nblk) {
+  constexpr int B = 1 << LOGB;
+  const int bid = block_id(plan);
+  const Slot s = locate<LOGB>(plan, bid);
+  const int b = threadIdx.x & (B - 1);
+  double acc = 0;
+  if (s.row >= 0) acc = row_dot<LOGB, double>(idx, val64, yp, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
+  acc = reduce_sub<LOGB>(acc, s.logS);
+  double q[3] = {0.0, 0.0, 0.0};
+  if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
+    const int j = s.row;
+    const double cj = c64[lc.at(j, b)], l = lo64[lb.at(j, b)], u = up64[lb.at(j, b)];
+    const double d = cj - acc;
+    const bool hl = l > -1e300, hu = u < 1e300;
+    double res = 0;
+    if (!hl && !hu) res = fabs(d);
+    else if (!hl) res = fmax(d, 0.0);
+    else if (!hu) res = fmax(-d, 0.0);
+    res *= cfac[j];
+    q[0] = res * res;
+    q[1] = cj * static_cast<double>(xp[(static_cast<size_t>(j) << LOGB) + b]);
+    if (d > 0 && hl) q[2] = d * l;
+    else if (d < 0 && hu) q[2] = d * u;
+  }
+  block_partials<LOGB, 3>(q, part, 2, nblk, bid);
+}
+
+// Check logic of one instance per workgroup, all FP64 (reference
+// pdhg.cpp:604-656): fixed-order sums of the partials, relative KKT,
+// convergence, the restart rule, the primal-weight update and new steps.
+__global__ __launch_bounds__(kWG) void k_finalize(Inst st, const double* __restrict__ colPart, int nblkC,
+                                                  const double* __restrict__ rowPart, int nblkR, RunScalars sc,
+                                                  int blockIters, int checkEvery) {
+  const int b = blockIdx.x, B = gridDim.x;
+  if (st.done[b]) return;
+  constexpr int NQ = 9;  // col 0..4, row 0..3
+  double v[NQ];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const double* p = colPart + (static_cast<size_t>(q) * B + b) * nblkC;
+    double s = 0;
+    for (int k = threadIdx.x; k < nblkC; k += kWG) s += p[k];
+    v[q] = s;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double* p = rowPart + (static_cast<size_t>(q) * B + b) * nblkR;
+    double s = 0;
+    for (int k = threadIdx.x; k < nblkR; k += kWG) s += p[k];
+    v[5 + q] = s;
+  }
+  __shared__ double sh[NQ][kWG / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double s = v[q];
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) sh[q][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double S[NQ];
+  for (int q = 0; q < NQ; ++q) S[q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+  const double dx2 = S[0], ax2 = S[1], dres2 = S[2], pobjS = S[3], dobjC = S[4];
+  const double pres2 = S[5], dobjR = S[6], dy2 = S[7], ay2 = S[8];
+  const long long it = st.iter[b] + blockIters - 1;
+  const double so = sc.beta * sc.gamma;
+  const double pobj = pobjS * so;
+  const double dobj = (dobjR + dobjC) * so;
+  const double relP = sqrt(pres2) / (1.0 + st.rhsNorm[b]);
+  const double relD = sqrt(dres2) / (1.0 + st.cNorm[b]);
+  const double gap = fabs(pobj - dobj) / (1.0 + fabs(pobj) + fabs(dobj));
+  double omega = st.omega[b];
+  CheckRecord r;
+  r.it = static_cast<double>(it);
+  r.relP = relP;
+  r.relD = relD;
+  r.gap = gap;
+  r.pobj = pobj + sc.offset;
+  r.dobj = dobj + sc.offset;
+  r.omega = omega;
+  r.restarted = 0;
+  st.iter[b] += blockIters;
+  if (relP <= sc.tol && relD <= sc.tol && gap <= sc.tol) {
+    st.done[b] = 1;
+    st.doneIter[b] = it;
+    st.rec[b] = r;
+    return;
+  }
+  long long since = st.since[b] + checkEvery;
+  const double fp = sqrt(omega * dx2 + dy2 / omega);
+  double fpR = st.fpR[b];
+  if (isinf(fpR)) fpR = fp;
+  const bool restart = fp <= 0.2 * fpR || (fp <= 0.8 * fpR && fp > st.fpPrev[b]) ||
+                       since >= static_cast<long long>(0.36 * static_cast<double>(it + 1));
+  double fpPrev = fp;
+  if (restart) {
+    const double dxn = sqrt(ax2), dyn = sqrt(ay2);
+    if (dxn > 1e-10 && dyn > 1e-10) omega = exp(0.5 * log(dyn / dxn) + 0.5 * log(omega));
+    fpR = sqrt(omega * dx2 + dy2 / omega);
+    fpPrev = INFINITY;
+    since = 0;
+    st.restarts[b] += 1;
+    st.omega[b] = omega;
+    st.tau[b] = sc.eta / omega;
+    st.sigma[b] = sc.eta * omega;
+    st.restartAt[b] = it + 1;
+    st.applyAt[b] = it + 1;
+    r.restarted = 1;
+    r.omega = omega;
+  }
+  st.since[b] = since;
+  st.fpR[b] = fpR;
+  st.fpPrev[b] = fpPrev;
+  st.rec[b] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Op-level kernels (the Backend vtable, reference pdhg.cpp:51-70)
+// ---------------------------------------------------------------------------
+template <typename T, int LOGB>
+__global__ __launch_bounds__(kWG) void k_spmv(RowPlan plan, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                              const T* __restrict__ val, const T* __restrict__ in,
+                                              T* __restrict__ out) {
+  constexpr int B = 1 << LOGB;
+  const int bid = block_id(plan);
+  const Slot s = locate<LOGB>(plan, bid);
+  const int b = threadIdx.x & (B - 1);
+  T acc = 0;
+  if (s.row >= 0) acc = row_dot<LOGB, T>(idx, val, in, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
+  acc = reduce_sub<LOGB>(acc, s.logS);
+  if (s.row >= 0 && s.sub == 0) out[(static_cast<size_t>(s.row) << LOGB) + b] = acc;
+}
+
+template <typename T>
+__global__ void k_op_primal(long long len, int k, const T* x, const T* aty, const T* c, const T* lo, const T* up,
+                            const double* tau, T* xp) {
+  for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < len;
+       t += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const T tv = static_cast<T>(tau[t % k]);
+    xp[t] = clamp_to<T>(x[t] - tv * (c[t] - aty[t]), lo[t], up[t]);
+  }
+}
+
+template <typename T>
+__global__ void k_op_dual(long long len, int k, const T* y, const T* axp, const T* ax, const T* rl, const T* ru,
+                          const double* sigma, T* yp) {
+  for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < len;
+       t += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const T s = static_cast<T>(sigma[t % k]);
+    const T v = y[t] - s * (static_cast<T>(2) * axp[t] - ax[t]);
+    yp[t] = v + s * clamp_to<T>(-v / s, rl[t], ru[t]);
+  }
+}
+
+template <typename T>
+__global__ void k_op_halpern(long long len, int k, const int* iter, double rho, T* z, const T* zp, const T* z0) {
+  for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < len;
+       t += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const double kk = iter[t % k];
+    const double lam = (kk + 1.0) / (kk + 2.0);
+    z[t] = static_cast<T>(lam) * (static_cast<T>(1.0 + rho) * zp[t] - static_cast<T>(rho) * z[t]) +
+           static_cast<T>(1.0 - lam) * z0[t];
+  }
+}
+
+__global__ void k_op_advance(int k, int* iter, const int* mask, int reset) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= k) return;
+  if (reset) {
+    if (mask[b]) iter[b] = 0;
+  } else {
+    iter[b] += 1;
+  }
+}
+
+template <typename T>
+__global__ void k_op_restart(long long len, int k, const int* mask, const T* src, T* d1, T* d2) {
+  for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < len;
+       t += static_cast<long long>(gridDim.x) * blockDim.x)
+    if (mask[t % k]) d1[t] = d2[t] = src[t];
+}
+
+// Partial sums partial[(b * g + blk) * 3 + s] over a grid (g, k), as the
+// reference's kkt_rows / kkt_cols / reduce_diff2 layout (kernels.cpp:112-192).
+__device__ inline void op_block_sum3(double a, double bq, double cq, double* out) {
+  __shared__ double sh[3][kWG / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double v[3] = {a, bq, cq};
+  for (int q = 0; q < 3; ++q) {
+    double s = v[q];
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) sh[q][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 3; ++q) out[q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_op_kkt_rows(int m, int k, const T* axp, const T* rl, const T* ru,
+                                                     const T* yp, const double* rfac, double* partial) {
+  const int b = blockIdx.y;
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int i = blockIdx.x * kWG + threadIdx.x; i < m; i += gridDim.x * kWG) {
+    const size_t t = static_cast<size_t>(i) * k + b;
+    const double a = axp[t], L = rl[t], U = ru[t], y = yp[t];
+    const double res = (fmax(L - a, 0.0) + fmax(a - U, 0.0)) * rfac[i];
+    s0 += res * res;
+    if (y > 0 && L > -1e300) s1 += y * L;
+    else if (y < 0 && U < 1e300) s1 += y * U;
+    s2 += y * y;
+  }
+  op_block_sum3(s0, s1, s2, partial + (static_cast<size_t>(b) * gridDim.x + blockIdx.x) * 3);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_op_kkt_cols(int n, int k, const T* xp, const T* c, const T* atyp,
+                                                     const T* lo, const T* up, const double* cfac,
+                                                     double* partial) {
+  const int b = blockIdx.y;
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int j = blockIdx.x * kWG + threadIdx.x; j < n; j += gridDim.x * kWG) {
+    const size_t t = static_cast<size_t>(j) * k + b;
+    const double cj = c[t], l = lo[t], u = up[t];
+    const double d = cj - static_cast<double>(atyp[t]);
+    const bool hl = l > -1e300, hu = u < 1e300;
+    double res = 0;
+    if (!hl && !hu) res = fabs(d);
+    else if (!hl) res = fmax(d, 0.0);
+    else if (!hu) res = fmax(-d, 0.0);
+    res *= cfac[j];
+    s0 += res * res;
+    s1 += cj * static_cast<double>(xp[t]);
+    if (d > 0 && hl) s2 += d * l;
+    else if (d < 0 && hu) s2 += d * u;
+  }
+  op_block_sum3(s0, s1, s2, partial + (static_cast<size_t>(b) * gridDim.x + blockIdx.x) * 3);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_op_diff2(int len, int k, const T* a, const T* bv, double* partial) {
+  const int b = blockIdx.y;
+  double s = 0;
+  for (int i = blockIdx.x * kWG + threadIdx.x; i < len; i += gridDim.x * kWG) {
+    const size_t t = static_cast<size_t>(i) * k + b;
+    const double d = static_cast<double>(a[t]) - static_cast<double>(bv[t]);
+    s += d * d;
+  }
+  op_block_sum3(s, 0.0, 0.0, partial + (static_cast<size_t>(b) * gridDim.x + blockIdx.x) * 3);
+}
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+int ilog2(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
+// Device buffer list owned by a run; freed together.
+struct Buffers {
+  std::vector<void*> ptrs;
+  ~Buffers() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  T* alloc(size_t count) {
+    void* p = nullptr;
+    HIP_OK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+    ptrs.push_back(p);
+    return static_cast<T*>(p);
+  }
+};
+
+// Host-side row mapping of one CSR matrix for B instances.
+struct PlanHost {
+  RowPlan plan{};
+  std::vector<int> perm;
+  int* dPerm = nullptr;
+  int blocks = 0;
+};
+
+PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms) {
+  PlanHost h;
+  const int rows = M.rows;
+  const int maxLogS = 6 - logB;  // B * S <= 64 lanes (one wavefront per row at most)
+  auto logs_of = [&](int len) {
+    const int per = std::max(1, ms.nnzPerLane);
+    int want = (len + per - 1) / per;
+    int l = ilog2(std::max(1, want));
+    return std::min(l, maxLogS);
+  };
+  int nb = 0;
+  if (ms.rowMap == 2 || maxLogS == 0) {
+    // One mapping: S from the mean row length.
+    const double mean = rows > 0 ? static_cast<double>(M.nnz()) / rows : 0;
+    const int l = logs_of(static_cast<int>(std::ceil(mean)));
+    h.plan.logS[0] = l;
+    h.plan.rowStart[0] = 0;
+    h.plan.rowCount[0] = rows;
+    nb = 1;
+  } else {
+    std::vector<int> cnt(maxLogS + 1, 0), bucket(rows);
+    for (int i = 0; i < rows; ++i) {
+      bucket[i] = logs_of(M.ptr[i + 1] - M.ptr[i]);
+      cnt[bucket[i]]++;
+    }
+    std::vector<int> start(maxLogS + 2, 0);
+    for (int l = 0; l <= maxLogS; ++l) start[l + 1] = start[l] + cnt[l];
+    h.perm.resize(rows);
+    std::vector<int> pos(start.begin(), start.end() - 1);
+    for (int i = 0; i < rows; ++i) h.perm[pos[bucket[i]]++] = i;
+    for (int l = 0; l <= maxLogS; ++l) {
+      if (cnt[l] == 0) continue;
+      h.plan.logS[nb] = l;
+      h.plan.rowStart[nb] = start[l];
+      h.plan.rowCount[nb] = cnt[l];
+      ++nb;
+    }
+    if (nb == 1) {  // a single bucket keeps the natural order
+      h.perm.clear();
+      h.plan.rowStart[0] = 0;
+    }
+    if (nb == 0) {
+      h.plan.logS[0] = 0;
+      h.plan.rowStart[0] = 0;
+      h.plan.rowCount[0] = 0;
+      nb = 1;
+    }
+  }
+  h.plan.nb = nb;
+  int blk = 0;
+  for (int q = 0; q < nb; ++q) {
+    h.plan.blkStart[q] = blk;
+    const int rowsPerWg = kWG >> (logB + h.plan.logS[q]);
+    blk += (h.plan.rowCount[q] + rowsPerWg - 1) / rowsPerWg;
+  }
+  h.plan.blkStart[nb] = blk;
+  h.blocks = std::max(blk, 1);
+  h.plan.xcd = ms.xcd ? 1 : 0;
+  h.plan.perm = nullptr;
+  return h;
+}
+
+std::mutex gInfoMutex;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Device probe (gpuAvailable / gpuInfoJson)
+// ---------------------------------------------------------------------------
+const DeviceInfo& device_info(int device) {
+  static std::vector<DeviceInfo> cache;
+  static std::vector<char> probed;
+  std::lock_guard<std::mutex> lk(gInfoMutex);
+  if (device < 0) device = 0;
+  if (static_cast<int>(cache.size()) <= device) {
+    cache.resize(device + 1);
+    probed.resize(device + 1, 0);
+  }
+  DeviceInfo& d = cache[device];
+  if (probed[device]) return d;
+  probed[device] = 1;
+  d.ordinal = device;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= device) {
+    d.reason = e != hipSuccess ? std::string("hipGetDeviceCount: ") + hipGetErrorString(e) : "no such HIP device";
+    return d;
+  }
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) {
+    d.reason = "hipGetDeviceProperties failed";
+    return d;
+  }
+  d.name = p.name;
+  d.arch = p.gcnArchName;
+  d.computeUnits = p.multiProcessorCount;
+  d.memoryBytes = p.totalGlobalMem;
+  // memoryClockRate is in kHz; bus width in bits; DDR.
+  d.peakBandwidthGBs = 2.0 * p.memoryClockRate * 1e3 * (p.memoryBusWidth / 8.0) / 1e9;
+  if (d.arch.rfind("gfx950", 0) != 0) {
+    d.reason = "device " + d.arch + " is not gfx950 (this build carries gfx950 code objects only)";
+    return d;
+  }
+  d.available = true;
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// DeviceRun implementation
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+struct Kern {
+  using ColFn = void (*)(RowPlan, const int*, const int*, const T*, const T*, T*, T*, T*, const T*, Ld, const T*,
+                         const T*, Ld, Inst, int, double, double*, int);
+  using RowFn = void (*)(RowPlan, const int*, const int*, const T*, const double*, const T*, T*, T*, T*, T*, T*, T*,
+                         const T*, const T*, const double*, const double*, Ld, const double*, Inst, int, double,
+                         double*, int);
+  using ColCheckFn = void (*)(RowPlan, const int*, const int*, const double*, const T*, const T*, const double*, Ld,
+                              const double*, const double*, Ld, const double*, Inst, double*, int);
+  using SpmvFn = void (*)(RowPlan, const int*, const int*, const T*, const T*, T*);
+  ColFn col, colChk;
+  RowFn row, rowChk;
+  ColCheckFn colcheck;
+  SpmvFn spmv;
+};
+
+template <typename T, int LOGB>
+Kern<T> kern_for() {
+  Kern<T> k;
+  k.col = k_col<T, LOGB, false>;
+  k.colChk = k_col<T, LOGB, true>;
+  k.row = k_row<T, LOGB, false>;
+  k.rowChk = k_row<T, LOGB, true>;
+  k.colcheck = k_colcheck<T, LOGB>;
+  k.spmv = k_spmv<T, LOGB>;
+  return k;
+}
+
+template <typename T>
+Kern<T> kern_table(int logB) {
+  switch (logB) {
+    case 0: return kern_for<T, 0>();
+    case 1: return kern_for<T, 1>();
+    case 2: return kern_for<T, 2>();
+    case 3: return kern_for<T, 3>();
+    case 4: return kern_for<T, 4>();
+    case 5: return kern_for<T, 5>();
+    case 6: return kern_for<T, 6>();
+  }
+  throw Error("batch width must be a power of two <= 64");
+}
+
+template <typename T>
+class RunImpl final : public DeviceRun {
+ public:
+  RunImpl(int device, const Prepared& P, const RunSetup& s) : s_(s), dev_(device) {
+    HIP_OK(hipSetDevice(device));
+    HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIP_OK(hipEventCreate(&ev0_));
+    HIP_OK(hipEventCreate(&ev1_));
+    m_ = P.m;
+    n_ = P.n;
+    B_ = s.B;
+    logB_ = ilog2(B_);
+    PDHG_REQUIRE((1 << logB_) == B_ && B_ <= 64, "batch width must be a power of two <= 64");
+    k_ = kern_table<T>(logB_);
+    nnz_ = P.A.nnz();
+    // Matrices: FP32 (or FP64) values for the iterations, FP64 for the checks.
+    aPtr_ = upload_int(P.A.ptr);
+    aIdx_ = upload_int(P.A.idx);
+    tPtr_ = upload_int(P.AT.ptr);
+    tIdx_ = upload_int(P.AT.idx);
+    aVal64_ = upload_dbl(P.A.val);
+    tVal64_ = upload_dbl(P.AT.val);
+    if (sizeof(T) == 8) {
+      aVal_ = reinterpret_cast<T*>(aVal64_);
+      tVal_ = reinterpret_cast<T*>(tVal64_);
+    } else {
+      aVal_ = upload_conv(P.A.val);
+      tVal_ = upload_conv(P.AT.val);
+    }
+    rfac_ = upload_dbl(P.rfac);
+    cfac_ = upload_dbl(P.cfac);
+    colPlan_ = build_plan(P.AT, logB_, s.colMap);
+    rowPlan_ = build_plan(P.A, logB_, s.rowMap);
+    for (PlanHost* h : {&colPlan_, &rowPlan_}) {
+      if (!h->perm.empty()) h->dPerm = upload_int(h->perm);
+      h->plan.perm = h->dPerm;
+    }
+    // Iterates (per instance) and case data (per instance or shared).
+    const size_t nB = static_cast<size_t>(n_) * B_, mB = static_cast<size_t>(m_) * B_;
+    lc_ = s.sharedCost ? Ld{1, 0} : Ld{B_, B_ - 1};
+    lb_ = s.sharedColBounds ? Ld{1, 0} : Ld{B_, B_ - 1};
+    lr_ = s.sharedRowBounds ? Ld{1, 0} : Ld{B_, B_ - 1};
+    const size_t nc = s.sharedCost ? n_ : nB, nbnd = s.sharedColBounds ? n_ : nB, mr = s.sharedRowBounds ? m_ : mB;
+    x_ = buf_.alloc<T>(nB);
+    xp_ = buf_.alloc<T>(nB);
+    x0_ = buf_.alloc<T>(nB);
+    y_ = buf_.alloc<T>(mB);
+    yp_ = buf_.alloc<T>(mB);
+    y0_ = buf_.alloc<T>(mB);
+    ax_ = buf_.alloc<T>(mB);
+    axp_ = buf_.alloc<T>(mB);
+    ax0_ = buf_.alloc<T>(mB);
+    c_ = buf_.alloc<T>(nc);
+    lo_ = buf_.alloc<T>(nbnd);
+    up_ = buf_.alloc<T>(nbnd);
+    rl_ = buf_.alloc<T>(mr);
+    ru_ = buf_.alloc<T>(mr);
+    c64_ = buf_.alloc<double>(nc);
+    lo64_ = buf_.alloc<double>(nbnd);
+    up64_ = buf_.alloc<double>(nbnd);
+    rl64_ = buf_.alloc<double>(mr);
+    ru64_ = buf_.alloc<double>(mr);
+    nc_ = nc;
+    nbnd_ = nbnd;
+    mr_ = mr;
+    colPart_ = buf_.alloc<double>(static_cast<size_t>(5) * B_ * colPlan_.blocks);
+    rowPart_ = buf_.alloc<double>(static_cast<size_t>(4) * B_ * rowPlan_.blocks);
+    // Instance state.
+    st_.tau = buf_.alloc<double>(B_);
+    st_.sigma = buf_.alloc<double>(B_);
+    st_.omega = buf_.alloc<double>(B_);
+    st_.fpR = buf_.alloc<double>(B_);
+    st_.fpPrev = buf_.alloc<double>(B_);
+    st_.iter = buf_.alloc<long long>(B_);
+    st_.since = buf_.alloc<long long>(B_);
+    st_.restartAt = buf_.alloc<long long>(B_);
+    st_.applyAt = buf_.alloc<long long>(B_);
+    st_.doneIter = buf_.alloc<long long>(B_);
+    st_.done = buf_.alloc<int>(B_);
+    st_.restarts = buf_.alloc<int>(B_);
+    cNorm_ = buf_.alloc<double>(B_);
+    rhsNorm_ = buf_.alloc<double>(B_);
+    st_.cNorm = cNorm_;
+    st_.rhsNorm = rhsNorm_;
+    st_.rec = buf_.alloc<CheckRecord>(B_);
+    HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&hostRec_), B_ * (sizeof(CheckRecord) + 2 * sizeof(int) + sizeof(long long)),
+                         hipHostMallocDefault));
+  }
+  ~RunImpl() override {
+    if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (hostRec_) (void)hipHostFree(hostRec_);
+    (void)hipEventDestroy(ev0_);
+    (void)hipEventDestroy(ev1_);
+    (void)hipStreamDestroy(stream_);
+  }
+
+  void upload(const double* c, const double* lo, const double* up, const double* rl, const double* ru, const double* x,
+              const double* y) override {
+    const size_t nB = static_cast<size_t>(n_) * B_, mB = static_cast<size_t>(m_) * B_;
+    put_both(c_, c64_, c, nc_);
+    put_both(lo_, lo64_, lo, nbnd_);
+    put_both(up_, up64_, up, nbnd_);
+    put_both(rl_, rl64_, rl, mr_);
+    put_both(ru_, ru64_, ru, mr_);
+    put(x_, x, nB);
+    put(x0_, x, nB);
+    put(xp_, x, nB);
+    put(y_, y, mB);
+    put(y0_, y, mB);
+    put(yp_, y, mB);
+    HIP_OK(hipStreamSynchronize(stream_));
+  }
+
+  void init_state(const RunScalars& sc, const double* omega, const double* cNorm, const double* rhsNorm,
+                  const int* done) override {
+    sc_ = sc;
+    std::vector<double> tau(B_), sigma(B_), inf(B_, INFINITY);
+    for (int b = 0; b < B_; ++b) {
+      tau[b] = sc.eta / omega[b];
+      sigma[b] = sc.eta * omega[b];
+    }
+    std::vector<long long> zero(B_, 0), neg(B_, -1);
+    std::vector<int> zi(B_, 0);
+    copy_h2d(st_.tau, tau.data(), B_);
+    copy_h2d(st_.sigma, sigma.data(), B_);
+    copy_h2d(st_.omega, omega, B_);
+    copy_h2d(st_.fpR, inf.data(), B_);
+    copy_h2d(st_.fpPrev, inf.data(), B_);
+    copy_h2d(st_.iter, zero.data(), B_);
+    copy_h2d(st_.since, zero.data(), B_);
+    copy_h2d(st_.restartAt, zero.data(), B_);
+    copy_h2d(st_.applyAt, neg.data(), B_);
+    copy_h2d(st_.doneIter, neg.data(), B_);
+    copy_h2d(st_.done, done, B_);
+    copy_h2d(st_.restarts, zi.data(), B_);
+    copy_h2d(cNorm_, cNorm, B_);
+    copy_h2d(rhsNorm_, rhsNorm, B_);
+    HIP_OK(hipStreamSynchronize(stream_));
+  }
+
+  void start() override {
+    launch_spmv(false, x_, ax_);
+    HIP_OK(hipMemcpyAsync(ax0_, ax_, sizeof(T) * m_ * static_cast<size_t>(B_), hipMemcpyDeviceToDevice, stream_));
+    HIP_OK(hipMemcpyAsync(axp_, ax_, sizeof(T) * m_ * static_cast<size_t>(B_), hipMemcpyDeviceToDevice, stream_));
+    HIP_OK(hipEventRecord(ev0_, stream_));
+  }
+
+  void run_block(int iters) override {
+    PDHG_REQUIRE(iters >= 1, "empty block");
+    if (s_.useGraph && iters == s_.checkEvery) {
+      if (!graphExec_) capture(iters);
+      HIP_OK(hipGraphLaunch(graphExec_, stream_));
+      graphLaunches++;
+      launches += 2 * iters + 2;
+    } else {
+      enqueue(iters);
+    }
+  }
+
+  void fetch(CheckRecord* rec, int* done, long long* doneIter, int* restarts) override {
+    char* h = reinterpret_cast<char*>(hostRec_);
+    CheckRecord* hr = hostRec_;
+    int* hd = reinterpret_cast<int*>(h + B_ * sizeof(CheckRecord));
+    int* hrs = hd + B_;
+    long long* hdi = reinterpret_cast<long long*>(hrs + B_);
+    HIP_OK(hipMemcpyAsync(hr, st_.rec, B_ * sizeof(CheckRecord), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipMemcpyAsync(hd, st_.done, B_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipMemcpyAsync(hrs, st_.restarts, B_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipMemcpyAsync(hdi, st_.doneIter, B_ * sizeof(long long), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    std::memcpy(rec, hr, B_ * sizeof(CheckRecord));
+    std::memcpy(done, hd, B_ * sizeof(int));
+    std::memcpy(restarts, hrs, B_ * sizeof(int));
+    std::memcpy(doneIter, hdi, B_ * sizeof(long long));
+    bytesDown += B_ * (sizeof(CheckRecord) + 2 * sizeof(int) + sizeof(long long));
+  }
+
+  void download(int which, double* out) override {
+    const size_t len = static_cast<size_t>(which == 0 ? n_ : m_) * B_;
+    std::vector<T> tmp(len);
+    HIP_OK(hipMemcpyAsync(tmp.data(), which == 0 ? xp_ : yp_, len * sizeof(T), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    for (size_t t = 0; t < len; ++t) out[t] = static_cast<double>(tmp[t]);
+    bytesDown += len * sizeof(T);
+  }
+
+  double stop() override {
+    HIP_OK(hipEventRecord(ev1_, stream_));
+    HIP_OK(hipEventSynchronize(ev1_));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    return ms / 1000.0;
+  }
+
+  void time_kernels(int reps, double* msCol, double* msRow) override {
+    // Launches exactly as inside an iteration; done flags make finished
+    // instances write nothing, so the run's results are not disturbed.
+    float ms = 0;
+    HIP_OK(hipEventRecord(ev0_, stream_));
+    for (int r = 0; r < reps; ++r) launch_col(false, 0);
+    HIP_OK(hipEventRecord(ev1_, stream_));
+    HIP_OK(hipEventSynchronize(ev1_));
+    HIP_OK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    *msCol = ms / reps;
+    HIP_OK(hipEventRecord(ev0_, stream_));
+    for (int r = 0; r < reps; ++r) launch_row(false, 0);
+    HIP_OK(hipEventRecord(ev1_, stream_));
+    HIP_OK(hipEventSynchronize(ev1_));
+    HIP_OK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    *msRow = ms / reps;
+  }
+
+  void kernel_bytes(double* colBytes, double* rowBytes) const override {
+    // Algorithmic bytes: matrix (value + column index + row pointer), one
+    // compulsory read of the gathered vector, and every streamed vector once.
+    const double vs = sizeof(T), nB = static_cast<double>(n_) * B_, mB = static_cast<double>(m_) * B_;
+    const double mat = (vs + 4.0) * nnz_;
+    const double nc = static_cast<double>(nc_), nb = static_cast<double>(nbnd_), mr = static_cast<double>(mr_);
+    *colBytes = mat + 4.0 * (n_ + 1) + vs * mB + vs * (2 * nB /*x, x0*/ + nc + 2 * nb) + vs * 2 * nB /*x, xp*/;
+    *rowBytes = mat + 4.0 * (m_ + 1) + vs * nB + vs * (4 * mB /*y, y0, ax, ax0*/ + 2 * mr) + vs * 2 * mB /*y, ax*/;
+  }
+
+ private:
+  template <typename U>
+  void copy_h2d(U* d, const U* h, size_t count) {
+    HIP_OK(hipMemcpyAsync(d, h, count * sizeof(U), hipMemcpyHostToDevice, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    bytesUp += count * sizeof(U);
+  }
+  int* upload_int(const std::vector<int>& v) {
+    int* d = buf_.alloc<int>(v.size());
+    copy_h2d(d, v.data(), v.size());
+    return d;
+  }
+  double* upload_dbl(const std::vector<double>& v) {
+    double* d = buf_.alloc<double>(v.size());
+    copy_h2d(d, v.data(), v.size());
+    return d;
+  }
+  T* upload_conv(const std::vector<double>& v) {
+    std::vector<T> t(v.begin(), v.end());
+    T* d = buf_.alloc<T>(v.size());
+    copy_h2d(d, t.data(), t.size());
+    return d;
+  }
+  void put(T* d, const double* h, size_t count) {
+    std::vector<T> t(h, h + count);
+    copy_h2d(d, t.data(), count);
+  }
+  void put_both(T* d, double* d64, const double* h, size_t count) {
+    put(d, h, count);
+    copy_h2d(d64, h, count);
+  }
+
+  void launch_spmv(bool transpose, const T* in, T* out) {
+    const PlanHost& p = transpose ? colPlan_ : rowPlan_;
+    hipLaunchKernelGGL(k_.spmv, dim3(p.blocks), dim3(kWG), 0, stream_, p.plan, transpose ? tPtr_ : aPtr_,
+                       transpose ? tIdx_ : aIdx_, transpose ? tVal_ : aVal_, in, out);
+    HIP_OK(hipGetLastError());
+    launches++;
+  }
+  void launch_col(bool check, int tloc) {
+    hipLaunchKernelGGL(check ? k_.colChk : k_.col, dim3(colPlan_.blocks), dim3(kWG), 0, stream_, colPlan_.plan, tPtr_,
+                       tIdx_, tVal_, y_, x_, xp_, x0_, c_, lc_, lo_, up_, lb_, st_, tloc, s_.rho, colPart_,
+                       colPlan_.blocks);
+  }
+  void launch_row(bool check, int tloc) {
+    hipLaunchKernelGGL(check ? k_.rowChk : k_.row, dim3(rowPlan_.blocks), dim3(kWG), 0, stream_, rowPlan_.plan, aPtr_,
+                       aIdx_, aVal_, aVal64_, xp_, y_, yp_, y0_, ax_, axp_, ax0_, rl_, ru_, rl64_, ru64_, lr_, rfac_,
+                       st_, tloc, s_.rho, rowPart_, rowPlan_.blocks);
+  }
+  void enqueue(int iters) {
+    for (int t = 0; t < iters; ++t) {
+      const bool chk = t == iters - 1;
+      launch_col(chk, t);
+      launch_row(chk, t);
+    }
+    hipLaunchKernelGGL(k_.colcheck, dim3(colPlan_.blocks), dim3(kWG), 0, stream_, colPlan_.plan, tPtr_, tIdx_,
+                       tVal64_, yp_, xp_, c64_, lc_, lo64_, up64_, lb_, cfac_, st_, colPart_, colPlan_.blocks);
+    hipLaunchKernelGGL(k_finalize, dim3(B_), dim3(kWG), 0, stream_, st_, colPart_, colPlan_.blocks, rowPart_,
+                       rowPlan_.blocks, sc_, iters, s_.checkEvery);
+    HIP_OK(hipGetLastError());
+    launches += 2 * iters + 2;
+  }
+  void capture(int iters) {
+    HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    enqueue(iters);
+    launches -= 2 * iters + 2;
+    HIP_OK(hipStreamEndCapture(stream_, &graph_));
+    HIP_OK(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0));
+  }
+
+  RunSetup s_;
+  int dev_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graphExec_ = nullptr;
+  Buffers buf_;
+  Kern<T> k_{};
+  int m_ = 0, n_ = 0, B_ = 1, logB_ = 0, nnz_ = 0;
+  int *aPtr_ = nullptr, *aIdx_ = nullptr, *tPtr_ = nullptr, *tIdx_ = nullptr;
+  T *aVal_ = nullptr, *tVal_ = nullptr;
+  double *aVal64_ = nullptr, *tVal64_ = nullptr, *rfac_ = nullptr, *cfac_ = nullptr;
+  PlanHost colPlan_, rowPlan_;
+  Ld lc_{1, 0}, lb_{1, 0}, lr_{1, 0};
+  size_t nc_ = 0, nbnd_ = 0, mr_ = 0;
+  T *x_, *xp_, *x0_, *y_, *yp_, *y0_, *ax_, *axp_, *ax0_, *c_, *lo_, *up_, *rl_, *ru_;
+  double *c64_, *lo64_, *up64_, *rl64_, *ru64_, *colPart_, *rowPart_, *cNorm_, *rhsNorm_;
+  Inst st_{};
+  RunScalars sc_{};
+  CheckRecord* hostRec_ = nullptr;
+};
+
+}  // namespace
+
+std::unique_ptr<DeviceRun> DeviceRun::create(int device, const Prepared& P, const RunSetup& s) {
+  const DeviceInfo& di = device_info(device);
+  if (!di.available) throw Error("GPU unavailable: " + di.reason);
+  if (s.precision == PDHG_FP64) return std::make_unique<RunImpl<double>>(device, P, s);
+  return std::make_unique<RunImpl<float>>(device, P, s);
+}
+
+// ---------------------------------------------------------------------------
+// Op-level backend: the reference Backend vtable over HIP (C-ABI below).
+// ---------------------------------------------------------------------------
+namespace {
+
+class OpBackendBase {
+ public:
+  virtual ~OpBackendBase() = default;
+  virtual void set(int v, const double* d) = 0;
+  virtual void get(int v, double* d) = 0;
+  virtual void spmv(int in, int out, bool transpose) = 0;
+  virtual void setSteps(const double* tau, const double* sigma) = 0;
+  virtual void primal() = 0;
+  virtual void dual() = 0;
+  virtual void halpern(double rho) = 0;
+  virtual void restart(const int* mask) = 0;
+  virtual void kkt(double* r3, double* c3) = 0;
+  virtual void diff2(int a, int b, double* out) = 0;
+  virtual double timeSpmv(bool transpose, int reps) = 0;
+};
+
+template <typename T>
+class OpBackend final : public OpBackendBase {
+ public:
+  OpBackend(int m, int n, const Csr& A, const Csr& AT, const double* rfac, const double* cfac, int k, int device)
+      : m_(m), n_(n), k_(k) {
+    HIP_OK(hipSetDevice(device));
+    HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIP_OK(hipEventCreate(&e0_));
+    HIP_OK(hipEventCreate(&e1_));
+    logB_ = ilog2(k);
+    PDHG_REQUIRE((1 << logB_) == k && k <= 64, "backend batch width must be a power of two <= 64");
+    kern_ = kern_table<T>(logB_);
+    aPtr_ = up(A.ptr);
+    aIdx_ = up(A.idx);
+    tPtr_ = up(AT.ptr);
+    tIdx_ = up(AT.idx);
+    aVal_ = upT(A.val);
+    tVal_ = upT(AT.val);
+    MapSetup ms;
+    rowPlan_ = build_plan(A, logB_, ms);
+    colPlan_ = build_plan(AT, logB_, ms);
+    for (PlanHost* h : {&rowPlan_, &colPlan_}) {
+      if (!h->perm.empty()) h->dPerm = up(h->perm);
+      h->plan.perm = h->dPerm;
+    }
+    rfac_ = buf_.alloc<double>(m);
+    cfac_ = buf_.alloc<double>(n);
+    HIP_OK(hipMemcpy(rfac_, rfac, sizeof(double) * m, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(cfac_, cfac, sizeof(double) * n, hipMemcpyHostToDevice));
+    for (int v = 0; v < PDHG_NV; ++v) {
+      const size_t len = len_of(v);
+      vec_[v] = buf_.alloc<T>(len);
+      HIP_OK(hipMemset(vec_[v], 0, len * sizeof(T)));
+    }
+    iter_ = buf_.alloc<int>(k);
+    mask_ = buf_.alloc<int>(k);
+    HIP_OK(hipMemset(iter_, 0, k * sizeof(int)));
+    tau_ = buf_.alloc<double>(k);
+    sigma_ = buf_.alloc<double>(k);
+    partial_ = buf_.alloc<double>(static_cast<size_t>(kRed) * k * 3);
+  }
+  ~OpBackend() override {
+    (void)hipEventDestroy(e0_);
+    (void)hipEventDestroy(e1_);
+    (void)hipStreamDestroy(stream_);
+  }
+  size_t len_of(int v) const { return static_cast<size_t>(v >= PDHG_V_Y ? m_ : n_) * k_; }
+  void set(int v, const double* d) override {
+    check_vec(v);
+    std::vector<T> t(d, d + len_of(v));
+    HIP_OK(hipMemcpy(vec_[v], t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void get(int v, double* d) override {
+    check_vec(v);
+    HIP_OK(hipStreamSynchronize(stream_));
+    std::vector<T> t(len_of(v));
+    HIP_OK(hipMemcpy(t.data(), vec_[v], t.size() * sizeof(T), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < t.size(); ++i) d[i] = t[i];
+  }
+  void spmv(int in, int out, bool transpose) override {
+    check_vec(in);
+    check_vec(out);
+    const PlanHost& p = transpose ? colPlan_ : rowPlan_;
+    hipLaunchKernelGGL(kern_.spmv, dim3(p.blocks), dim3(kWG), 0, stream_, p.plan, transpose ? tPtr_ : aPtr_,
+                       transpose ? tIdx_ : aIdx_, transpose ? tVal_ : aVal_, vec_[in], vec_[out]);
+    HIP_OK(hipGetLastError());
+  }
+  void setSteps(const double* tau, const double* sigma) override {
+    HIP_OK(hipStreamSynchronize(stream_));
+    HIP_OK(hipMemcpy(tau_, tau, k_ * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(sigma_, sigma, k_ * sizeof(double), hipMemcpyHostToDevice));
+  }
+  void primal() override {
+    const long long len = static_cast<long long>(n_) * k_;
+    hipLaunchKernelGGL(k_op_primal<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, k_, vec_[PDHG_V_X],
+                       vec_[PDHG_V_ATY], vec_[PDHG_V_C], vec_[PDHG_V_LO], vec_[PDHG_V_UP], tau_, vec_[PDHG_V_XP]);
+    HIP_OK(hipGetLastError());
+  }
+  void dual() override {
+    const long long len = static_cast<long long>(m_) * k_;
+    hipLaunchKernelGGL(k_op_dual<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, k_, vec_[PDHG_V_Y],
+                       vec_[PDHG_V_AXP], vec_[PDHG_V_AX], vec_[PDHG_V_RL], vec_[PDHG_V_RU], sigma_, vec_[PDHG_V_YP]);
+    HIP_OK(hipGetLastError());
+  }
+  void halpern(double rho) override {
+    auto one = [&](int rows, int z, int zp, int z0) {
+      const long long len = static_cast<long long>(rows) * k_;
+      hipLaunchKernelGGL(k_op_halpern<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, k_, iter_, rho, vec_[z],
+                         vec_[zp], vec_[z0]);
+    };
+    one(n_, PDHG_V_X, PDHG_V_XP, PDHG_V_X0);
+    one(m_, PDHG_V_Y, PDHG_V_YP, PDHG_V_Y0);
+    one(m_, PDHG_V_AX, PDHG_V_AXP, PDHG_V_AX0);
+    hipLaunchKernelGGL(k_op_advance, dim3(1), dim3(64), 0, stream_, k_, iter_, mask_, 0);
+    HIP_OK(hipGetLastError());
+  }
+  void restart(const int* mask) override {
+    HIP_OK(hipStreamSynchronize(stream_));
+    HIP_OK(hipMemcpy(mask_, mask, k_ * sizeof(int), hipMemcpyHostToDevice));
+    auto cp = [&](int rows, int src, int d1, int d2) {
+      const long long len = static_cast<long long>(rows) * k_;
+      hipLaunchKernelGGL(k_op_restart<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, k_, mask_, vec_[src],
+                         vec_[d1], vec_[d2]);
+    };
+    cp(n_, PDHG_V_XP, PDHG_V_X, PDHG_V_X0);
+    cp(m_, PDHG_V_YP, PDHG_V_Y, PDHG_V_Y0);
+    cp(m_, PDHG_V_AXP, PDHG_V_AX, PDHG_V_AX0);
+    hipLaunchKernelGGL(k_op_advance, dim3(1), dim3(64), 0, stream_, k_, iter_, mask_, 1);
+    HIP_OK(hipGetLastError());
+  }
+  void kkt(double* r3, double* c3) override {
+    std::fill(r3, r3 + 3 * k_, 0.0);
+    std::fill(c3, c3 + 3 * k_, 0.0);
+    std::vector<double> host;
+    if (m_ > 0) {
+      const int g = red_blocks(m_);
+      hipLaunchKernelGGL(k_op_kkt_rows<T>, dim3(g, k_), dim3(kWG), 0, stream_, m_, k_, vec_[PDHG_V_AXP],
+                         vec_[PDHG_V_RL], vec_[PDHG_V_RU], vec_[PDHG_V_YP], rfac_, partial_);
+      fetch_sum3(g, r3);
+    }
+    const int g = red_blocks(n_);
+    hipLaunchKernelGGL(k_op_kkt_cols<T>, dim3(g, k_), dim3(kWG), 0, stream_, n_, k_, vec_[PDHG_V_XP],
+                       vec_[PDHG_V_C], vec_[PDHG_V_ATYP], vec_[PDHG_V_LO], vec_[PDHG_V_UP], cfac_, partial_);
+    fetch_sum3(g, c3);
+  }
+  void diff2(int a, int b, double* out) override {
+    check_vec(a);
+    check_vec(b);
+    const int len = a >= PDHG_V_Y ? m_ : n_;
+    std::fill(out, out + k_, 0.0);
+    if (len == 0) return;
+    const int g = red_blocks(len);
+    hipLaunchKernelGGL(k_op_diff2<T>, dim3(g, k_), dim3(kWG), 0, stream_, len, k_, vec_[a], vec_[b], partial_);
+    std::vector<double> r3(3 * k_, 0.0);
+    fetch_sum3(g, r3.data());
+    for (int q = 0; q < k_; ++q) out[q] = r3[3 * q];
+  }
+  double timeSpmv(bool transpose, int reps) override {
+    const int in = transpose ? PDHG_V_Y : PDHG_V_X, out = transpose ? PDHG_V_ATY : PDHG_V_AX;
+    spmv(in, out, transpose);  // warm
+    HIP_OK(hipEventRecord(e0_, stream_));
+    for (int r = 0; r < reps; ++r) spmv(in, out, transpose);
+    HIP_OK(hipEventRecord(e1_, stream_));
+    HIP_OK(hipEventSynchronize(e1_));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, e0_, e1_));
+    return ms / reps;
+  }
+
+ private:
+  static constexpr int kRed = 256;
+  void check_vec(int v) const { PDHG_REQUIRE(v >= 0 && v < PDHG_NV, "vector id out of range"); }
+  static int grid(long long len) { return static_cast<int>(std::max(1LL, std::min(8192LL, (len + kWG - 1) / kWG))); }
+  static int red_blocks(int len) { return std::max(1, std::min(kRed, (len + kWG - 1) / kWG)); }
+  void fetch_sum3(int g, double* acc) {
+    std::vector<double> host(static_cast<size_t>(g) * k_ * 3);
+    HIP_OK(hipMemcpyAsync(host.data(), partial_, host.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    for (int b = 0; b < k_; ++b)  // fixed order, as the reference host sum (pdhg.cpp:346-348)
+      for (int q = 0; q < g; ++q)
+        for (int s = 0; s < 3; ++s) acc[3 * b + s] += host[(static_cast<size_t>(b) * g + q) * 3 + s];
+  }
+  int* up(const std::vector<int>& v) {
+    int* d = buf_.alloc<int>(v.size());
+    HIP_OK(hipMemcpy(d, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+    return d;
+  }
+  T* upT(const std::vector<double>& v) {
+    std::vector<T> t(v.begin(), v.end());
+    T* d = buf_.alloc<T>(v.size());
+    HIP_OK(hipMemcpy(d, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+  }
+
+  int m_, n_, k_, logB_ = 0;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t e0_ = nullptr, e1_ = nullptr;
+  Buffers buf_;
+  Kern<T> kern_{};
+  int *aPtr_, *aIdx_, *tPtr_, *tIdx_;
+  T *aVal_, *tVal_;
+  PlanHost rowPlan_, colPlan_;
+  double *rfac_, *cfac_, *tau_, *sigma_, *partial_;
+  T* vec_[PDHG_NV];
+  int *iter_, *mask_;
+};
+
+thread_local std::string tErr;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    tErr = e.what();
+    return -1;
+  }
+}
+
+}  // namespace
+
+const char* device_last_error() { return tErr.c_str(); }
+
+}  // namespace pdhgamd
+
+// ---------------------------------------------------------------------------
+// C-ABI: op-level backend
+// ---------------------------------------------------------------------------
+struct pdhg_backend {
+  std::unique_ptr<pdhgamd::OpBackendBase> impl;
+};
+
+extern "C" {
+
+void pdhg_set_last_error(const char* msg);
+
+pdhg_backend* pdhg_backend_create(int m, int n, const int* a_ptr, const int* a_idx, const double* a_val,
+                                  const int* t_ptr, const int* t_idx, const double* t_val, const double* rfac,
+                                  const double* cfac, int k, int precision, int device) {
+  try {
+    using namespace pdhgamd;
+    const DeviceInfo& di = device_info(device);
+    if (!di.available) throw Error("GPU unavailable: " + di.reason);
+    PDHG_REQUIRE(m >= 0 && n >= 0 && k >= 1, "bad sizes");
+    Csr A, AT;
+    A.rows = m;
+    A.cols = n;
+    A.ptr.assign(a_ptr, a_ptr + m + 1);
+    A.idx.assign(a_idx, a_idx + a_ptr[m]);
+    A.val.assign(a_val, a_val + a_ptr[m]);
+    AT.rows = n;
+    AT.cols = m;
+    AT.ptr.assign(t_ptr, t_ptr + n + 1);
+    AT.idx.assign(t_idx, t_idx + t_ptr[n]);
+    AT.val.assign(t_val, t_val + t_ptr[n]);
+    PDHG_REQUIRE(A.nnz() == AT.nnz(), "A and A^T nonzero counts differ");
+    for (int e = 0; e < A.nnz(); ++e) PDHG_REQUIRE(A.idx[e] >= 0 && A.idx[e] < n, "A column index out of range");
+    for (int e = 0; e < AT.nnz(); ++e) PDHG_REQUIRE(AT.idx[e] >= 0 && AT.idx[e] < m, "A^T column index out of range");
+    auto* h = new pdhg_backend;
+    if (precision == PDHG_FP64) h->impl = std::make_unique<OpBackend<double>>(m, n, A, AT, rfac, cfac, k, device);
+    else h->impl = std::make_unique<OpBackend<float>>(m, n, A, AT, rfac, cfac, k, device);
+    return h;
+  } catch (const std::exception& e) {
+    pdhg_set_last_error(e.what());
+    return nullptr;
+  }
+}
+
+void pdhg_backend_destroy(pdhg_backend* be) { delete be; }
+
+#define PDHG_BE_CALL(body)                                                  \
+  do {                                                                      \
+    if (!be) {                                                              \
+      pdhg_set_last_error("null backend");                                  \
+      return -1;                                                            \
+    }                                                                       \
+    if (pdhgamd::guarded([&] { body; }) != 0) {                             \
+      pdhg_set_last_error(pdhgamd::device_last_error());                    \
+      return -1;                                                            \
+    }                                                                       \
+    return 0;                                                               \
+  } while (0)
+
+int pdhg_backend_set(pdhg_backend* be, int vec, const double* data) { PDHG_BE_CALL(be->impl->set(vec, data)); }
+int pdhg_backend_get(pdhg_backend* be, int vec, double* out) { PDHG_BE_CALL(be->impl->get(vec, out)); }
+int pdhg_backend_spmv(pdhg_backend* be, int in, int out) { PDHG_BE_CALL(be->impl->spmv(in, out, false)); }
+int pdhg_backend_spmvT(pdhg_backend* be, int in, int out) { PDHG_BE_CALL(be->impl->spmv(in, out, true)); }
+int pdhg_backend_set_steps(pdhg_backend* be, const double* tau, const double* sigma) {
+  PDHG_BE_CALL(be->impl->setSteps(tau, sigma));
+}
+int pdhg_backend_primal_step(pdhg_backend* be) { PDHG_BE_CALL(be->impl->primal()); }
+int pdhg_backend_dual_step(pdhg_backend* be) { PDHG_BE_CALL(be->impl->dual()); }
+int pdhg_backend_halpern(pdhg_backend* be, double rho) { PDHG_BE_CALL(be->impl->halpern(rho)); }
+int pdhg_backend_restart(pdhg_backend* be, const int* mask) { PDHG_BE_CALL(be->impl->restart(mask)); }
+int pdhg_backend_kkt(pdhg_backend* be, double* rows3k, double* cols3k) { PDHG_BE_CALL(be->impl->kkt(rows3k, cols3k)); }
+int pdhg_backend_diff2(pdhg_backend* be, int a, int b, double* out) { PDHG_BE_CALL(be->impl->diff2(a, b, out)); }
+int pdhg_backend_time_spmv(pdhg_backend* be, int transpose, int reps, double* ms) {
+  PDHG_BE_CALL(*ms = be->impl->timeSpmv(transpose != 0, reps < 1 ? 1 : reps));
+}
+
+}  // extern "C"
