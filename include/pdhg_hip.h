@@ -96,6 +96,8 @@ typedef struct pdhg_options {
   int xcd_remap;               /* -1 = auto, 0 = off, 1 = contiguous rows per XCD */
   double time_limit;           /* seconds, <= 0: none (Deadline, src/util/timer.h:22) */
   int device;                  /* HIP device ordinal (0) */
+  int kernel_timing_reps;      /* > 0: after the solve, time this many launches of each
+                                  iteration kernel with HIP events (stats *_kernel_ms) */
 } pdhg_options;
 
 /* PdhgStats (src/pdhg/pdhg.h:44-56) plus engine counters. */
@@ -110,6 +112,8 @@ typedef struct pdhg_stats {
   int precision;
   double spmv_bytes_per_iter;  /* algorithmic bytes of the 2 SpMVs per iteration */
   double iter_bytes;           /* algorithmic bytes of one fused iteration */
+  double col_kernel_ms, row_kernel_ms;        /* mean launch time (kernel_timing_reps > 0) */
+  double col_kernel_bytes, row_kernel_bytes;  /* algorithmic bytes per launch */
   /* Optional per-check trace of instance 0 (caller-owned, may be NULL). */
   double* trace;               /* records of 8 doubles: it, relP, relD, gap, pobj, dobj, omega, restarted */
   int trace_capacity;
@@ -153,6 +157,13 @@ int pdhg_solve_batch(const pdhg_lp* base, const pdhg_case* cases, int k, const p
  * one through solvePdhg, src/pdhg/pdhg.h:73). */
 int pdhg_solve_packed(const pdhg_lp* lps, int count, const pdhg_options* opts, pdhg_result* results,
                       pdhg_stats* stats);
+
+/* prepare (src/pdhg/pdhg.cpp:424-459): Ruiz + Pock-Chambolle factors R[m],
+ * C[n] (powers of two), scalars[3] = {beta, gamma, 1.01 ||A_scaled||_2} and
+ * the CSR of the scaled matrix (a_ptr[m+1], a_idx[nnz], a_val[nnz]).  Any
+ * output may be NULL.  Host FP64, bit-identical to the reference. */
+int pdhg_prepare(const pdhg_lp* lp, int ruiz_iterations, int pock_chambolle, int rescale, double* R, double* C,
+                 double* scalars, int* a_ptr, int* a_idx, double* a_val);
 
 /* ---- op-level backend (the Backend vtable, src/pdhg/pdhg.cpp:51-70) ----- */
 typedef struct pdhg_backend pdhg_backend;

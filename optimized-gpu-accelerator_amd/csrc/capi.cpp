@@ -13,6 +13,7 @@
 #include "certify.h"
 #include "device.h"
 #include "engine.h"
+#include "prepare.h"
 
 namespace pdhgamd {
 Model gen_random_lp(uint64_t seed, int m, int n, double perCol, double rowSigma);
@@ -142,6 +143,25 @@ int pdhg_solve_packed(const pdhg_lp* lps, int count, const pdhg_options* opts, p
   (void)stats;
   gLastError = "pdhg_solve_packed: packed uneven batches are not available in this build";
   return -1;
+}
+
+int pdhg_prepare(const pdhg_lp* lp, int ruiz_iterations, int pock_chambolle, int rescale, double* R, double* C,
+                 double* scalars, int* a_ptr, int* a_idx, double* a_val) {
+  return guard([&] {
+    PDHG_REQUIRE(lp, "null argument");
+    const pdhgamd::Model md = pdhgamd::model_from(lp);
+    const pdhgamd::Prepared P = pdhgamd::prepare(md, ruiz_iterations, pock_chambolle != 0, rescale != 0);
+    if (R) std::copy(P.R.begin(), P.R.end(), R);
+    if (C) std::copy(P.C.begin(), P.C.end(), C);
+    if (scalars) {
+      scalars[0] = P.beta;
+      scalars[1] = P.gamma;
+      scalars[2] = P.normA;
+    }
+    if (a_ptr) std::copy(P.A.ptr.begin(), P.A.ptr.end(), a_ptr);
+    if (a_idx) std::copy(P.A.idx.begin(), P.A.idx.end(), a_idx);
+    if (a_val) std::copy(P.A.val.begin(), P.A.val.end(), a_val);
+  });
 }
 
 // ---- certifier -----------------------------------------------------------

@@ -221,6 +221,11 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       }
     }
     kernelSeconds += run->stop();
+    if (o.kernel_timing_reps > 0 && first == 0) {
+      // After the loop: every instance is released so each launch does the
+      // full work of an iteration (the results were already reached).
+      run->time_kernels(o.kernel_timing_reps, &acc.col_kernel_ms, &acc.row_kernel_ms);
+    }
     maxItersRun = std::max(maxItersRun, it);
 
     // ---- extraction (src/pdhg/pdhg.cpp:571-587, 668-680) ----
@@ -261,6 +266,8 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     double colB = 0, rowB = 0;
     run->kernel_bytes(&colB, &rowB);
     acc.iter_bytes = colB + rowB;
+    acc.col_kernel_bytes = colB;
+    acc.row_kernel_bytes = rowB;
     const double vs = o.precision == PDHG_FP64 ? 8.0 : 4.0;
     acc.spmv_bytes_per_iter = 2 * ((vs + 4.0) * P.A.nnz()) + 4.0 * (m + 1) + 4.0 * (n + 1) +
                               2 * vs * static_cast<double>(B) * (m + n);

@@ -929,8 +929,15 @@ class RunImpl final : public DeviceRun {
   }
 
   void time_kernels(int reps, double* msCol, double* msRow) override {
-    // Launches exactly as inside an iteration; done flags make finished
-    // instances write nothing, so the run's results are not disturbed.
+    // Launches exactly as inside an iteration.  Called after the results were
+    // read: every instance is released (no done flag, no pending restart) so
+    // each launch streams and writes its full data.
+    std::vector<int> zeros(B_, 0);
+    std::vector<long long> none(B_, -1);
+    copy_h2d(st_.done, zeros.data(), B_);
+    copy_h2d(st_.applyAt, none.data(), B_);
+    launch_col(false, 0);
+    launch_row(false, 0);
     float ms = 0;
     HIP_OK(hipEventRecord(ev0_, stream_));
     for (int r = 0; r < reps; ++r) launch_col(false, 0);

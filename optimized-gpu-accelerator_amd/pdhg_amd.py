@@ -44,7 +44,8 @@ class Options(C.Structure):
                 ("use_gpu", C.c_int), ("ruiz_iterations", C.c_int), ("pock_chambolle", C.c_int),
                 ("rescale", C.c_int), ("reflection", C.c_double), ("log_interval", C.c_int),
                 ("precision", C.c_int), ("use_graph", C.c_int), ("lanes_per_row", C.c_int), ("row_map", C.c_int),
-                ("xcd_remap", C.c_int), ("time_limit", C.c_double), ("device", C.c_int)]
+                ("xcd_remap", C.c_int), ("time_limit", C.c_double), ("device", C.c_int),
+                ("kernel_timing_reps", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -55,6 +56,8 @@ class Stats(C.Structure):
                 ("bytes_downloaded", C.c_longlong), ("kernel_launches", C.c_longlong),
                 ("graph_launches", C.c_longlong), ("spectral_norm", C.c_double), ("batch", C.c_int),
                 ("precision", C.c_int), ("spmv_bytes_per_iter", C.c_double), ("iter_bytes", C.c_double),
+                ("col_kernel_ms", C.c_double), ("row_kernel_ms", C.c_double), ("col_kernel_bytes", C.c_double),
+                ("row_kernel_bytes", C.c_double),
                 ("trace", _dp), ("trace_capacity", C.c_int), ("trace_length", C.c_int),
                 ("backend", C.c_char * 32), ("device_name", C.c_char * 64)]
 
@@ -69,7 +72,7 @@ class ResultStruct(C.Structure):
 # Every symbol include/pdhg_hip.h declares (checked by the CPU test suite).
 EXPORTS = [
     "pdhg_last_error", "pdhg_default_options", "pdhg_gpu_available", "pdhg_gpu_info_json",
-    "pdhg_solve", "pdhg_solve_batch", "pdhg_solve_packed",
+    "pdhg_solve", "pdhg_solve_batch", "pdhg_solve_packed", "pdhg_prepare",
     "pdhg_backend_create", "pdhg_backend_destroy", "pdhg_backend_set", "pdhg_backend_get",
     "pdhg_backend_spmv", "pdhg_backend_spmvT", "pdhg_backend_set_steps", "pdhg_backend_primal_step",
     "pdhg_backend_dual_step", "pdhg_backend_halpern", "pdhg_backend_restart", "pdhg_backend_kkt",
@@ -104,6 +107,7 @@ def lib() -> C.CDLL:
         L.pdhg_solve.argtypes = [C.POINTER(LpStruct), C.POINTER(Options), C.POINTER(ResultStruct), C.POINTER(Stats)]
         L.pdhg_solve_batch.argtypes = [C.POINTER(LpStruct), C.POINTER(CaseStruct), C.c_int, C.POINTER(Options),
                                        C.POINTER(ResultStruct), C.POINTER(Stats)]
+        L.pdhg_prepare.argtypes = [C.POINTER(LpStruct), C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _ip, _ip, _dp]
         L.pdhg_safe_dual_bound.argtypes = [C.POINTER(LpStruct), _dp, _dp, _ip, _dp]
         L.pdhg_certify.argtypes = [C.POINTER(LpStruct), _dp, _dp, C.c_double, _dp]
         L.pdhg_certify_batch.argtypes = [C.POINTER(LpStruct), C.POINTER(CaseStruct), C.c_int, _dp, _dp, _dp,
@@ -335,6 +339,16 @@ def solve_batch(lp: LP, costs: np.ndarray | None = None, opts: Options | None = 
     _check(lib().pdhg_solve_batch(C.byref(s), cases, k, C.byref(opts), res, C.byref(st)), "pdhg_solve_batch")
     out = [_result(res[b], xs[b], ys[b]) for b in range(k)]
     return out, stats_dict(st), (tr[: st.trace_length] if tr is not None else None)
+
+
+def prepare(lp: LP, ruiz: int = 10, pc: int = 1, rescale: int = 1) -> dict:
+    """Host set-up of the engine: scaling factors, beta, gamma, ||A||, scaled CSR."""
+    R, Cs, sc = np.zeros(lp.m), np.zeros(lp.n), np.zeros(3)
+    ptr, idx, val = np.zeros(lp.m + 1, np.int32), np.zeros(lp.nnz, np.int32), np.zeros(lp.nnz)
+    s = lp.struct()
+    _check(lib().pdhg_prepare(C.byref(s), ruiz, pc, rescale, _d(R), _d(Cs), _d(sc), _i(ptr), _i(idx), _d(val)),
+           "prepare")
+    return dict(R=R, C=Cs, beta=sc[0], gamma=sc[1], normA=sc[2], ptr=ptr, idx=idx, val=val)
 
 
 def safe_dual_bound(lp: LP, y: np.ndarray):

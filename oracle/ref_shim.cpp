@@ -13,7 +13,10 @@
 #include <string>
 #include <vector>
 
+#include <cmath>
+
 #include "cert/certifier.h"
+#include "mip/propagate.h"
 #include "pdhg/pdhg.h"
 #include "util/parallel.h"
 
@@ -143,6 +146,35 @@ int ref_safe_dual_bound(int m, int n, const int* start, const int* index, const 
     *bound = r.bound;
     *infiniteTerms = r.infiniteTerms;
     *costPerturbation = r.costPerturbation;
+    return 0;
+  } catch (const std::exception& e) {
+    gErr = e.what();
+    return -1;
+  }
+}
+
+// Family certification bound of the reference: impliedBounds
+// (src/parametric/parametric.cpp:62-73, which is file-local there, so its
+// four statements are spelled out here around the reference's own Propagator)
+// followed by safeDualBound, plus the objective offset (:516-517).
+int ref_family_bound(int m, int n, const int* start, const int* index, const double* value, const double* c,
+                     const double* l, const double* u, const double* L, const double* U, double offset,
+                     const double* y, double* bound) {
+  try {
+    pramana::Model md = makeModel(m, n, start, index, value, c, l, u, L, U, offset);
+    std::vector<double> lo = md.colLower, up = md.colUpper;
+    pramana::Propagator p;
+    p.setup(md);
+    std::vector<double> l2 = lo, u2 = up;
+    p.propagate(l2, u2, 30);
+    auto fin = [](double v) { return std::fabs(v) < pramana::kInfBoundThreshold; };
+    for (size_t j = 0; j < lo.size(); ++j) {
+      if (fin(l2[j]) && (!fin(lo[j]) || l2[j] > lo[j])) lo[j] = l2[j] - 1e-6 * (1 + std::fabs(l2[j]));
+      if (fin(u2[j]) && (!fin(up[j]) || u2[j] < up[j])) up[j] = u2[j] + 1e-6 * (1 + std::fabs(u2[j]));
+    }
+    pramana::SafeBoundResult sb = pramana::safeDualBound(md.A, md.colCost, lo, up, md.rowLower, md.rowUpper,
+                                                         std::vector<double>(y, y + m));
+    *bound = sb.bound + offset;
     return 0;
   } catch (const std::exception& e) {
     gErr = e.what();

@@ -151,3 +151,13 @@ def ref_safe_bound(lp, y):
     rc = ref().ref_safe_dual_bound(*_lp_args(lp), _d(y), C.byref(b), C.byref(inf), C.byref(pert))
     assert rc == 0
     return b.value, inf.value, pert.value
+
+
+def ref_family_bound(lp, y, cost=None):
+    c = lp.c if cost is None else np.ascontiguousarray(cost, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    b = C.c_double()
+    rc = ref().ref_family_bound(lp.m, lp.n, _i(lp.start), _i(lp.index), _d(lp.value), _d(c), _d(lp.l), _d(lp.u),
+                                _d(lp.L), _d(lp.U), C.c_double(lp.offset), _d(y), C.byref(b))
+    assert rc == 0, ref().ref_last_error()
+    return b.value

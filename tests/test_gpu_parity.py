@@ -1,0 +1,129 @@
+"""GPU parity of the HIP engine against the FP64 oracle (runs on an MI355X).
+
+Bar: SpMV within FP32 rounding of the FP64 product; full solves reach the
+requested relative KKT (re-evaluated independently in FP64 on the host) and
+their objectives agree with the oracle's and with the closed-form optimum.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle_lib as O
+import pdhg_amd as P
+from conftest import load_netlib
+
+pytestmark = pytest.mark.gpu
+
+
+def scaled_csr(lp):
+    pr = O.oracle_prepare(lp)
+    A = sp.csr_matrix((pr["val"], pr["idx"], pr["ptr"]), shape=(lp.m, lp.n))
+    A.sort_indices()
+    AT = A.T.tocsr()
+    AT.sort_indices()
+    rfac = pr["beta"] / pr["R"]
+    cfac = pr["gamma"] / pr["C"]
+    return A, AT, rfac, cfac
+
+
+def host_kkt(lp, x, y, cost=None):
+    """Relative KKT of an unscaled (x, y) in FP64 on the host, with the
+    reference's definitions applied to the original problem (scaling factors
+    1, beta = gamma = 1): relP, relD, gap."""
+    c = lp.c if cost is None else cost
+    A = lp.scipy_csc()
+    ax = A @ x
+    fin = lambda v: np.abs(v) < 1e20
+    res = np.maximum(np.where(fin(lp.L), lp.L - ax, 0), 0) + np.maximum(np.where(fin(lp.U), ax - lp.U, 0), 0)
+    d = c - A.T @ y
+    hl, hu = fin(lp.l), fin(lp.u)
+    dres = np.where(~hl & ~hu, np.abs(d), np.where(~hl, np.maximum(d, 0), np.where(~hu, np.maximum(-d, 0), 0)))
+    pobj = c @ x + lp.offset
+    dobj = (np.sum(np.where((y > 0) & fin(lp.L), y * np.where(fin(lp.L), lp.L, 0), 0))
+            + np.sum(np.where((y < 0) & fin(lp.U), y * np.where(fin(lp.U), lp.U, 0), 0))
+            + np.sum(np.where((d > 0) & hl, d * np.where(hl, lp.l, 0), 0))
+            + np.sum(np.where((d < 0) & hu, d * np.where(hu, lp.u, 0), 0)) + lp.offset)
+    bn = np.sqrt(np.sum(np.where(fin(lp.L), lp.L, 0) ** 2) + np.sum(np.where(fin(lp.U) & (lp.U != lp.L), lp.U, 0) ** 2))
+    relP = np.linalg.norm(res) / (1 + bn)
+    relD = np.linalg.norm(dres) / (1 + np.linalg.norm(c))
+    gap = abs(pobj - dobj) / (1 + abs(pobj) + abs(dobj))
+    return relP, relD, gap, pobj
+
+
+@pytest.mark.parametrize("prec", [P.FP32, P.FP64])
+@pytest.mark.parametrize("k", [1, 8, 64])
+def test_spmv_matches_fp64_product(gpu, prec, k):
+    lp = P.gen_random_lp(2, 3000, 6000, 5.12, 1.0)  # log-normal row lengths
+    A, AT, rfac, cfac = scaled_csr(lp)
+    be = P.Backend(A, AT, rfac, cfac, k=k, precision=prec)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((lp.n, k))
+    y = rng.standard_normal((lp.m, k))
+    be.set(P.V_X, x)
+    be.set(P.V_Y, y)
+    be.spmv(P.V_X, P.V_AX)
+    be.spmvT(P.V_Y, P.V_ATY)
+    ax = be.get(P.V_AX).reshape(lp.m, k)
+    aty = be.get(P.V_ATY).reshape(lp.n, k)
+    if prec == P.FP32:  # the product of the FP32-rounded operands, accumulated in FP32
+        xr, yr = x.astype(np.float32).astype(np.float64), y.astype(np.float32).astype(np.float64)
+        Ar = sp.csr_matrix((A.data.astype(np.float32).astype(np.float64), A.indices, A.indptr), shape=A.shape)
+        ATr = Ar.T.tocsr()
+        tol_ax = 4e-7 * 64 * (abs(Ar) @ abs(xr)) + 1e-30
+        tol_aty = 4e-7 * 64 * (abs(ATr) @ abs(yr)) + 1e-30
+        ex, ey = Ar @ xr, ATr @ yr
+    else:
+        tol_ax = 1e-14 * (abs(A) @ abs(x)) + 1e-300
+        tol_aty = 1e-14 * (abs(AT) @ abs(y)) + 1e-300
+        ex, ey = A @ x, AT @ y
+    assert np.all(np.abs(ax - ex) <= tol_ax)
+    assert np.all(np.abs(aty - ey) <= tol_aty)
+    be.close()
+
+
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32])
+def test_solve_random_lp_matches_oracle(gpu, prec):
+    lp = P.gen_random_lp(1, 1000, 2000, 4.0, 0.0)
+    obj_star = float(lp.c @ lp.xstar)
+    ref, _, _, trace = O.oracle_solve(lp, tol=1e-6, trace_cap=4096)
+    r, st, tr = P.solve(lp, P.default_options(tolerance=1e-6, precision=prec), trace_capacity=4096)
+    print(f"prec={prec} gpu iters {r.iterations} oracle {ref[0]['iterations']} obj {r.objective:.12g} "
+          f"oracle {ref[0]['objective']:.12g} closed form {obj_star:.12g} restarts {r.restarts}/{ref[0]['restarts']}")
+    assert r.status == P.OPTIMAL
+    assert r.relative_kkt <= 1e-6
+    relP, relD, gap, pobj = host_kkt(lp, r.x, r.y)
+    assert max(relP, relD, gap) <= 1.5e-6, (relP, relD, gap)
+    scale = 1 + abs(obj_star)
+    assert abs(r.objective - ref[0]["objective"]) <= 1e-5 * scale
+    assert abs(r.objective - obj_star) <= 1e-5 * scale
+    if prec == P.FP64:  # same arithmetic up to FMA / summation order: same restart decisions
+        assert abs(r.iterations - ref[0]["iterations"]) <= 4 * 64
+        n = min(5, len(tr), len(trace))
+        for t in range(n):
+            assert tr[t][0] == trace[t]["it"]
+            for col, key in ((1, "relP"), (2, "relD"), (3, "gap")):
+                assert abs(tr[t][col] - trace[t][key]) <= 1e-6 * abs(trace[t][key]) + 1e-14
+
+
+def test_sc50a_known_optimum(gpu):
+    """Reference test pdhg_cpu_matches_optimum_and_gpu_agrees (tests/cpp/test_solvers.cpp:112-130)."""
+    lp, opt = load_netlib("sc50a")
+    r, _, _ = P.solve(lp, P.default_options(tolerance=1e-7, precision=P.FP64))
+    assert r.status == P.OPTIMAL
+    assert abs(r.objective - opt) <= 1e-5 * (1 + abs(opt))
+    lb, _, _ = P.safe_dual_bound(lp, r.y)
+    assert lb <= opt + 1e-9
+
+
+def test_afiro_batch_equals_individual(gpu):
+    """Reference test pdhg_batch_equals_individual_solves (tests/cpp/test_solvers.cpp:132-149)."""
+    lp, _ = load_netlib("afiro")
+    costs = np.stack([lp.c.copy() for _ in range(4)])
+    for k in range(4):
+        costs[k, 1] = -0.4 + 0.1 * k
+    ref, _, _, _ = O.oracle_solve(lp, costs, tol=1e-7)
+    for prec in (P.FP64, P.FP32):
+        res, _, _ = P.solve_batch(lp, costs, P.default_options(tolerance=1e-7, precision=prec))
+        for k in range(4):
+            assert res[k].status == P.OPTIMAL
+            assert abs(res[k].objective - ref[k]["objective"]) <= 1e-5 * (1 + abs(ref[k]["objective"]))

@@ -1,0 +1,148 @@
+"""CPU-side tests of the product library: the C-ABI surface, the host set-up
+(bit-identical to the reference), the certifier and the generators.  No GPU
+compute is called here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import pdhg_amd as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="compiled reference (oracle/_ref) not built")
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "pdhg_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pdhg_[a-zA-Z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = P.lib()
+    syms = header_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert sorted(P.EXPORTS) == syms
+
+
+def test_no_gpu_means_loud_failure():
+    if P.gpu_available():
+        pytest.skip("a GPU is present")
+    info = P.gpu_info()
+    assert not info["available"] and info["reason"]
+    lp = P.gen_random_lp(1, 50, 100)
+    with pytest.raises(RuntimeError, match="GPU unavailable"):
+        P.solve(lp)
+
+
+def test_bad_input_is_rejected():
+    lp = P.gen_random_lp(1, 50, 100)
+    bad = P.LP(lp.m, lp.n, lp.start, lp.index.copy(), lp.value, lp.c, lp.l, lp.u, lp.L, lp.U)
+    bad.index[3] = lp.m + 5
+    with pytest.raises(RuntimeError, match="row index out of range"):
+        P.safe_dual_bound(bad, np.zeros(lp.m))
+
+
+@pytest.mark.parametrize("gen", ["c0", "c1", "netlib", "refinery"])
+def test_prepare_bitwise_equals_oracle(gen):
+    from conftest import load_netlib
+
+    if gen == "c0":
+        lp = P.gen_random_lp(1, 1000, 2000, 4.0, 0.0)
+    elif gen == "c1":
+        lp = P.gen_random_lp(2, 5000, 10000, 5.12, 1.0)
+    elif gen == "netlib":
+        lp, _ = load_netlib("share2b")
+    else:
+        lp = P.gen_refinery(3, periods=2, products=6, qualities=5)
+    a, b = P.prepare(lp), O.oracle_prepare(lp)
+    for key in ("R", "C", "ptr", "idx", "val"):
+        assert np.array_equal(a[key], b[key]), key
+    for key in ("beta", "gamma", "normA"):
+        assert a[key] == b[key], key
+
+
+@needs_ref
+def test_prepare_norm_equals_reference():
+    lp = P.gen_random_lp(1, 1000, 2000, 4.0, 0.0)
+    r, _, _ = O.ref_solve(lp, tol=1e-6, max_iter=64)
+    assert P.prepare(lp)["normA"] == r["spectral_norm"]
+
+
+def test_generator_random_lp_closed_form():
+    lp = P.gen_random_lp(1, 1000, 2000, 4.0, 0.0)
+    A = lp.scipy_csc()
+    assert 7800 <= lp.nnz <= 8000
+    assert np.allclose(A @ lp.xstar, lp.L, rtol=0, atol=1e-12) and np.array_equal(lp.L, lp.U)
+    s = lp.c - A.T @ lp.ystar
+    assert np.all(s >= -1e-12)
+    assert np.all(np.minimum(s, lp.xstar) <= 1e-12)  # complementarity
+    assert np.sum(lp.xstar == 0) == 1000
+    assert np.all(lp.l == 0) and np.all(np.isinf(lp.u))
+
+
+def test_generator_lognormal_rows():
+    lp = P.gen_random_lp(2, 50000, 100000, 5.12, 1.0)
+    assert abs(lp.nnz - 5.12e5) / 5.12e5 < 0.03
+    rows = np.diff(lp.scipy_csc().tocsr().indptr)
+    assert rows.max() > 8 * rows.mean()  # heavy tail of a log-normal (sigma = 1)
+
+
+def test_generator_refinery_shape():
+    lp = P.gen_refinery(3)
+    assert abs(lp.nnz - 5.6e5) / 5.6e5 < 0.05
+    assert np.all(np.isfinite(lp.u)) and np.all(lp.l == 0)  # every column bounded
+    A = lp.scipy_csc()
+    assert np.all(lp.L <= 1e-12) and np.all(lp.U >= -1e-12)  # x = 0 is feasible
+    vals = np.abs(A.data)
+    assert vals.max() / vals.min() > 1e8  # deliberately bad scaling
+    th, cs = P.gen_family(lp, 4, 64, 0.3)
+    assert th.shape == (64, 4) and np.all(np.abs(th) <= 0.3)
+    changed = np.nonzero(np.any(cs != lp.c, axis=0))[0]
+    assert 0 < changed.size <= 4 * 12
+    assert np.allclose(P.family_costs(lp, th), cs)
+
+
+def test_safe_bound_matches_oracle_bitwise():
+    lp = P.gen_random_lp(5, 400, 800, 4.0, 0.0)
+    rng = np.random.default_rng(3)
+    for _ in range(10):
+        y = lp.ystar + 1e-3 * rng.standard_normal(lp.m)
+        assert P.safe_dual_bound(lp, y) == O.oracle_safe_bound(lp, y)
+
+
+def test_safe_bound_known_answer():
+    from test_oracle import two_var_lp
+
+    lp = two_var_lp()
+    lb, inf, _ = P.safe_dual_bound(lp, np.array([0.4, 0.2]))
+    assert inf == 0 and lb <= 2.8 and 2.8 - lb <= 1e-12
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        assert P.safe_dual_bound(lp, rng.uniform(-0.5, 1.5, 2))[0] <= 2.8 + 1e-12
+
+
+@needs_ref
+def test_family_certificate_matches_reference():
+    lp = P.gen_refinery(3, periods=2, products=6, qualities=5)
+    rng = np.random.default_rng(4)
+    y = rng.standard_normal(lp.m) * 1e-2
+    cert = P.certify(lp, None, y, 0.0)
+    assert cert["bound"] == O.ref_family_bound(lp, y)
+
+
+def test_certificate_rejects_wrong_objective():
+    from test_oracle import two_var_lp
+
+    lp = two_var_lp()
+    good = P.certify(lp, np.array([1.6, 1.2]), np.array([0.4, 0.2]), 2.8)
+    assert good["accepted"] and good["primal_violation"] == 0
+    bad = P.certify(lp, np.array([2.0, 2.0]), np.array([0.4, 0.2]), 4.0)
+    assert not bad["accepted"]
+    out = P.certify_batch(lp, None, np.array([[1.6, 1.2], [2.0, 2.0]]), np.array([[0.4, 0.2], [0.4, 0.2]]),
+                          np.array([2.8, 4.0]), threads=2)
+    assert out[0, 3] == 1.0 and out[1, 3] == 0.0
