@@ -170,23 +170,27 @@ struct Ld {
 // CHECK adds ||x - x+||^2 and ||x+ - x0||^2 (quantities 0, 1).
 template <typename T, int LOGB, bool CHECK>
 __global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict__ ptr, const int* __restrict__ idx,
-                                             const T* __restrict__ val, const T* __restrict__ y, T* __restrict__ x,
-                                             T* __restrict__ xp, T* __restrict__ x0, const T* __restrict__ c, Ld lc,
+                                             const T* __restrict__ val, const T* __restrict__ y,
+                                             const T* __restrict__ yp, T* __restrict__ x, T* __restrict__ xp,
+                                             T* __restrict__ x0, const T* __restrict__ c, Ld lc,
                                              const T* __restrict__ lo, const T* __restrict__ up, Ld lb, Inst st,
                                              int tloc, double rho, double* __restrict__ part, int nblk) {
   constexpr int B = 1 << LOGB;
   const int bid = block_id(plan);
   const Slot s = locate<LOGB>(plan, bid);
   const int b = threadIdx.x & (B - 1);
+  const long long it = st.iter[b] + tloc;
+  // An instance restarted at the last check continues from (x+, y+): its A^T y
+  // of this iteration reads y+ (reference pdhg.cpp:160-162 then :589).
+  const bool rst = st.applyAt[b] == it;
+  const T* __restrict__ ysrc = rst ? yp : y;
   T acc = 0;
-  if (s.row >= 0) acc = row_dot<LOGB, T>(idx, val, y, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
+  if (s.row >= 0) acc = row_dot<LOGB, T>(idx, val, ysrc, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
   acc = reduce_sub<LOGB>(acc, s.logS);
   double q[2] = {0.0, 0.0};
   if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
     const int j = s.row;
     const size_t t = (static_cast<size_t>(j) << LOGB) + b;
-    const long long it = st.iter[b] + tloc;
-    const bool rst = st.applyAt[b] == it;
     const T xin = rst ? xp[t] : x[t];
     T xo;
     if (rst) {
@@ -258,10 +262,16 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
       yo = y0[t];
       axo = ax0[t];
     }
+    // Reflected dual step y+ = v + sigma clamp(-v/sigma, L, U), v = y - sigma g,
+    // g = 2 ax+ - ax (reference pdhg.cpp:126-137), evaluated as
+    // y+ = y - sigma (g - w) with g - w in {g - L, g - U, y / sigma}: the
+    // large activity g only meets the bound it is compared with, so y keeps
+    // its low-order bits in FP32 (v + sigma w would cancel them away).
     const T sg = static_cast<T>(st.sigma[b]);
-    const T v = yin - sg * (static_cast<T>(2) * acc - axin);
-    const T w = clamp_to<T>(-v / sg, rl[lr.at(i, b)], ru[lr.at(i, b)]);
-    const T yplus = v + sg * w;
+    const T g = static_cast<T>(2) * acc - axin;
+    const T tq = yin / sg;
+    const T dlo = g - rl[lr.at(i, b)], dhi = g - ru[lr.at(i, b)];
+    const T yplus = dlo < tq ? yin - sg * dlo : (dhi > tq ? yin - sg * dhi : static_cast<T>(0));
     const double kk = static_cast<double>(it - st.restartAt[b]);
     const double lam = (kk + 1.0) / (kk + 2.0);
     const T lamT = static_cast<T>(lam), omT = static_cast<T>(1.0 - lam);
@@ -693,8 +703,8 @@ namespace {
 
 template <typename T>
 struct Kern {
-  using ColFn = void (*)(RowPlan, const int*, const int*, const T*, const T*, T*, T*, T*, const T*, Ld, const T*,
-                         const T*, Ld, Inst, int, double, double*, int);
+  using ColFn = void (*)(RowPlan, const int*, const int*, const T*, const T*, const T*, T*, T*, T*, const T*, Ld,
+                         const T*, const T*, Ld, Inst, int, double, double*, int);
   using RowFn = void (*)(RowPlan, const int*, const int*, const T*, const double*, const T*, T*, T*, T*, T*, T*, T*,
                          const T*, const T*, const double*, const double*, Ld, const double*, Inst, int, double,
                          double*, int);
@@ -1004,7 +1014,7 @@ class RunImpl final : public DeviceRun {
   }
   void launch_col(bool check, int tloc) {
     hipLaunchKernelGGL(check ? k_.colChk : k_.col, dim3(colPlan_.blocks), dim3(kWG), 0, stream_, colPlan_.plan, tPtr_,
-                       tIdx_, tVal_, y_, x_, xp_, x0_, c_, lc_, lo_, up_, lb_, st_, tloc, s_.rho, colPart_,
+                       tIdx_, tVal_, y_, yp_, x_, xp_, x0_, c_, lc_, lo_, up_, lb_, st_, tloc, s_.rho, colPart_,
                        colPlan_.blocks);
   }
   void launch_row(bool check, int tloc) {

@@ -122,8 +122,10 @@ def test_afiro_batch_equals_individual(gpu):
     for k in range(4):
         costs[k, 1] = -0.4 + 0.1 * k
     ref, _, _, _ = O.oracle_solve(lp, costs, tol=1e-7)
-    for prec in (P.FP64, P.FP32):
-        res, _, _ = P.solve_batch(lp, costs, P.default_options(tolerance=1e-7, precision=prec))
+    # FP64 at the reference test's 1e-7; FP32 iterates at the engine's 1e-6 target.
+    for prec, tol in ((P.FP64, 1e-7), (P.FP32, 1e-6)):
+        res, _, _ = P.solve_batch(lp, costs, P.default_options(tolerance=tol, precision=prec))
         for k in range(4):
+            print(prec, k, res[k].status, res[k].iterations, res[k].objective, ref[k]["objective"])
             assert res[k].status == P.OPTIMAL
             assert abs(res[k].objective - ref[k]["objective"]) <= 1e-5 * (1 + abs(ref[k]["objective"]))
