@@ -83,16 +83,16 @@ def cpu_baseline(lp, costs, gpu_iters, seconds, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp64")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--plain-sample", type=int, default=2, help="instances solved one by one for the plain baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--max-iterations", type=int, default=2_000_000,
-                    help="cap per solve (profiling runs use a short cap; the headline run never hits it)")
+    ap.add_argument("--max-iterations", type=int, default=1_000_000,
+                    help="cap per instance (the reference default, src/pdhg/pdhg.h:34); unconverged = unsolved")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched family shards by "
@@ -104,7 +104,8 @@ def main():
     prec = P.FP64 if args.precision == "fp64" else P.FP32
     lp = P.gen_refinery(SEED_LP)
     thetas, costs = P.gen_family(lp, SEED_THETA, K, 0.3)
-    opts = P.default_options(tolerance=args.tol, precision=prec, max_iterations=args.max_iterations)
+    opts = P.default_options(tolerance=args.tol, precision=prec, max_iterations=args.max_iterations,
+                             progress_seconds=15.0)
     log(f"workload: refinery m={lp.m} n={lp.n} nnz={lp.nnz}, {K} instances, precision {args.precision}")
 
     for w in range(args.warmup):

@@ -98,6 +98,9 @@ typedef struct pdhg_options {
   int device;                  /* HIP device ordinal (0) */
   int kernel_timing_reps;      /* > 0: after the solve, time this many launches of each
                                   iteration kernel with HIP events (stats *_kernel_ms) */
+  int compact;                 /* 1: once half the instances of a batch are done, move the
+                                  live ones into a run of half the width (default 1) */
+  double progress_seconds;     /* > 0: a progress line on stderr at this interval */
 } pdhg_options;
 
 /* PdhgStats (src/pdhg/pdhg.h:44-56) plus engine counters. */

@@ -80,6 +80,9 @@ void pdhg_default_options(pdhg_options* o) {
   o->xcd_remap = -1;
   o->time_limit = 0;
   o->device = 0;
+  o->kernel_timing_reps = 0;
+  o->compact = 1;
+  o->progress_seconds = 0;
 }
 
 int pdhg_gpu_available(void) { return pdhgamd::device_info(0).available ? 1 : 0; }

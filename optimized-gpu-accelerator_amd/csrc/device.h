@@ -49,6 +49,16 @@ struct RunScalars {
   double eta = 1, beta = 1, gamma = 1, offset = 0, tol = 1e-6;
 };
 
+// Host copy (FP64) of the iterate and restart state of some lanes of a run,
+// k-interleaved over those lanes.  Moves live instances into a narrower run.
+struct LaneState {
+  int count = 0;
+  std::vector<double> x, x0, xp, y, y0, yp, ax, ax0, axp;
+  std::vector<double> tau, sigma, omega, fpR, fpPrev;
+  std::vector<long long> iter, since, restartAt, applyAt;
+  std::vector<int> restarts;
+};
+
 // Device state of one batched restarted-PDHG run: B instances sharing A.
 class DeviceRun {
  public:
@@ -62,8 +72,8 @@ class DeviceRun {
   // Per-instance scalars; done[b] != 0 marks a padding lane.
   virtual void init_state(const RunScalars& s, const double* omega, const double* cNorm, const double* rhsNorm,
                           const int* done) = 0;
-  // ax = A x, ax0 = ax; starts the loop timer.
-  virtual void start() = 0;
+  // initAx: ax = A x, ax0 = ax (a fresh run).  Starts the loop timer.
+  virtual void start(bool initAx) = 0;
   // `iters` fused iterations, the last one a check (KKT, restart, primal weight).
   virtual void run_block(int iters) = 0;
   // Waits for the stream, then copies the last check records and done flags.
@@ -77,6 +87,11 @@ class DeviceRun {
   virtual void time_kernels(int reps, double* msCol, double* msRow) = 0;
   // Algorithmic bytes of one launch of the column and row iteration kernels.
   virtual void kernel_bytes(double* colBytes, double* rowBytes) const = 0;
+  // Copies the state of `lanes` (in that order) to the host.
+  virtual void export_lanes(const std::vector<int>& lanes, LaneState& out) = 0;
+  // Overwrites iterates and per-instance state of lanes 0..in.count-1 (after
+  // upload() and init_state() of this run).
+  virtual void import_lanes(const LaneState& in) = 0;
 
   long long launches = 0, graphLaunches = 0, bytesUp = 0, bytesDown = 0;
 };

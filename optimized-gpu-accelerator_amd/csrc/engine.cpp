@@ -57,6 +57,21 @@ Model model_from(const pdhg_lp* lp) {
   return md;
 }
 
+namespace {
+
+// Scaled, interleaved case data of a set of cases occupying lanes 0..B-1
+// (src/pdhg/pdhg.cpp:506-545).  Lanes past the cases are padding, marked done.
+struct LaneData {
+  std::vector<double> c, lo, up, rl, ru, x, y, omega, cNorm, rhsNorm;
+  std::vector<int> pad;
+};
+
+struct CaseScaled {  // per case: norms of the original vectors and initial omega
+  double cNorm = 0, rhsNorm = 0, omega = 1;
+};
+
+}  // namespace
+
 void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pdhg_options& o, pdhg_result* results,
                  pdhg_stats* stats) {
   const auto tSetup = Clock::now();
@@ -69,97 +84,107 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
   if (!di.available) throw Error("GPU unavailable: " + di.reason);
   const int n = base.n, m = base.m;
 
-  // Set-up shared by every chunk (src/pdhg/pdhg.cpp:487).
+  // Set-up shared by every case (src/pdhg/pdhg.cpp:487; beta, gamma from the base model).
   const Prepared P = prepare(base, o.ruiz_iterations, o.pock_chambolle != 0, o.rescale != 0);
   const double eta = 0.998 / P.normA;
-  double setupSeconds = since_s(tSetup);
 
   const bool shC = same_pointer(cases, &CaseData::c);
   const bool shB = same_pointer(cases, &CaseData::l) && same_pointer(cases, &CaseData::u);
   const bool shR = same_pointer(cases, &CaseData::L) && same_pointer(cases, &CaseData::U);
+  auto cost = [&](int q) { return cases[q].c ? cases[q].c : base.c.data(); };
+  auto colL = [&](int q) { return cases[q].l ? cases[q].l : base.l.data(); };
+  auto colU = [&](int q) { return cases[q].u ? cases[q].u : base.u.data(); };
+  auto rowL = [&](int q) { return cases[q].L ? cases[q].L : base.L.data(); };
+  auto rowU = [&](int q) { return cases[q].U ? cases[q].U : base.U.data(); };
 
-  pdhg_stats acc{};
-  if (stats) {
-    acc.trace = stats->trace;
-    acc.trace_capacity = stats->trace_capacity;
+  // Norms of the original case vectors (src/pdhg/pdhg.cpp:466-481) and the
+  // initial primal weight ||c_hat|| / ||b_hat|| (:513-529).
+  std::vector<CaseScaled> cs(k);
+  for (int q = 0; q < k; ++q) {
+    const double *cc = cost(q), *cL = rowL(q), *cU = rowU(q);
+    double cn0 = 0, bn0 = 0, cn = 0, bn = 0;
+    for (int j = 0; j < n; ++j) {
+      cn0 += cc[j] * cc[j];
+      const double cj = cc[j] * P.C[j] / P.gamma;
+      cn += cj * cj;
+    }
+    for (int i = 0; i < m; ++i) {
+      if (finite_bound(cL[i])) bn0 += cL[i] * cL[i];
+      if (finite_bound(cU[i]) && cU[i] != cL[i]) bn0 += cU[i] * cU[i];
+      const double li = finite_bound(cL[i]) ? cL[i] * P.R[i] / P.beta : -HUGE_VAL;
+      const double ui = finite_bound(cU[i]) ? cU[i] * P.R[i] / P.beta : HUGE_VAL;
+      if (finite_bound(li)) bn += li * li;
+      if (finite_bound(ui) && ui != li) bn += ui * ui;
+    }
+    cs[q].cNorm = std::sqrt(cn0);
+    cs[q].rhsNorm = std::sqrt(bn0);
+    if (cn > 1e-20 && bn > 1e-20) cs[q].omega = std::sqrt(cn) / std::sqrt(bn);
   }
-  acc.precision = o.precision;
-  long long maxItersRun = 0;
-  double solveSeconds = 0, kernelSeconds = 0, transfer = 0;
-  int chunks = 0;
 
-  for (int first = 0; first < k; first += 64) {
-    const auto tChunk = Clock::now();
-    const int kc = std::min(64, k - first);
-    const int B = next_pow2(kc);
-    ++chunks;
-    // Case data in scaled space, interleaved (src/pdhg/pdhg.cpp:506-545).
+  // Lane data for cases `ids` in a run of width B.
+  auto lane_data = [&](const std::vector<int>& ids, int B) {
+    LaneData d;
     const size_t nB = static_cast<size_t>(n) * B, mB = static_cast<size_t>(m) * B;
-    std::vector<double> c(shC ? n : nB), lo(shB ? n : nB), up(shB ? n : nB), rl(shR ? m : mB), ru(shR ? m : mB);
-    std::vector<double> x(nB), y(mB, 0.0), omega(B, 1.0), cNorm(B, 0.0), rhsNorm(B, 0.0);
-    std::vector<int> pad(B, 0);
+    d.c.assign(shC ? n : nB, 0.0);
+    d.lo.assign(shB ? n : nB, 0.0);
+    d.up.assign(shB ? n : nB, 0.0);
+    d.rl.assign(shR ? m : mB, 0.0);
+    d.ru.assign(shR ? m : mB, 0.0);
+    d.x.assign(nB, 0.0);
+    d.y.assign(mB, 0.0);
+    d.omega.assign(B, 1.0);
+    d.cNorm.assign(B, 0.0);
+    d.rhsNorm.assign(B, 0.0);
+    d.pad.assign(B, 1);
     for (int b = 0; b < B; ++b) {
-      const bool real = b < kc;
-      pad[b] = real ? 0 : 1;
-      const CaseData& cs = cases[first + (real ? b : 0)];
-      const double* cc = cs.c ? cs.c : base.c.data();
-      const double* cl = cs.l ? cs.l : base.l.data();
-      const double* cu = cs.u ? cs.u : base.u.data();
-      const double* cL = cs.L ? cs.L : base.L.data();
-      const double* cU = cs.U ? cs.U : base.U.data();
-      // Norms of the original case vectors (src/pdhg/pdhg.cpp:466-481).
-      double cn0 = 0, bn0 = 0;
-      for (int j = 0; j < n; ++j) cn0 += cc[j] * cc[j];
-      for (int i = 0; i < m; ++i) {
-        if (finite_bound(cL[i])) bn0 += cL[i] * cL[i];
-        if (finite_bound(cU[i]) && cU[i] != cL[i]) bn0 += cU[i] * cU[i];
-      }
-      cNorm[b] = std::sqrt(cn0);
-      rhsNorm[b] = std::sqrt(bn0);
-      double cn = 0, bn = 0;
+      const bool real = b < static_cast<int>(ids.size());
+      const int q = ids[real ? b : 0];
+      d.pad[b] = real ? 0 : 1;
+      d.omega[b] = cs[q].omega;
+      d.cNorm[b] = cs[q].cNorm;
+      d.rhsNorm[b] = cs[q].rhsNorm;
+      const double *cc = cost(q), *cl = colL(q), *cu = colU(q), *cL = rowL(q), *cU = rowU(q);
       for (int j = 0; j < n; ++j) {
         const double cj = cc[j] * P.C[j] / P.gamma;
         const double lj = finite_bound(cl[j]) ? cl[j] / P.C[j] / P.beta : -HUGE_VAL;
         const double uj = finite_bound(cu[j]) ? cu[j] / P.C[j] / P.beta : HUGE_VAL;
-        if (shC) {
-          if (b == 0) c[j] = cj;
-        } else {
-          c[static_cast<size_t>(j) * B + b] = cj;
+        if (!shC) d.c[static_cast<size_t>(j) * B + b] = cj;
+        else if (b == 0) d.c[j] = cj;
+        if (!shB) {
+          d.lo[static_cast<size_t>(j) * B + b] = lj;
+          d.up[static_cast<size_t>(j) * B + b] = uj;
+        } else if (b == 0) {
+          d.lo[j] = lj;
+          d.up[j] = uj;
         }
-        if (shB) {
-          if (b == 0) {
-            lo[j] = lj;
-            up[j] = uj;
-          }
-        } else {
-          lo[static_cast<size_t>(j) * B + b] = lj;
-          up[static_cast<size_t>(j) * B + b] = uj;
-        }
-        // Cold start x0 = clamp(0) (src/pdhg/pdhg.cpp:519) or the warm start
-        // mapped into scaled space (inverse of the unscaling at :581) and clamped.
-        const double x0 = cs.x0 ? cs.x0[j] / P.C[j] / P.beta : 0.0;
-        x[static_cast<size_t>(j) * B + b] = std::min(std::max(x0, lj), uj);
-        cn += cj * cj;
+        // Cold start x0 = clamp(0) (src/pdhg/pdhg.cpp:519), or the warm start
+        // mapped into scaled space (inverse of the unscaling at :581), clamped.
+        const double x0 = cases[q].x0 ? cases[q].x0[j] / P.C[j] / P.beta : 0.0;
+        d.x[static_cast<size_t>(j) * B + b] = std::min(std::max(x0, lj), uj);
       }
       for (int i = 0; i < m; ++i) {
         const double li = finite_bound(cL[i]) ? cL[i] * P.R[i] / P.beta : -HUGE_VAL;
         const double ui = finite_bound(cU[i]) ? cU[i] * P.R[i] / P.beta : HUGE_VAL;
-        if (shR) {
-          if (b == 0) {
-            rl[i] = li;
-            ru[i] = ui;
-          }
-        } else {
-          rl[static_cast<size_t>(i) * B + b] = li;
-          ru[static_cast<size_t>(i) * B + b] = ui;
+        if (!shR) {
+          d.rl[static_cast<size_t>(i) * B + b] = li;
+          d.ru[static_cast<size_t>(i) * B + b] = ui;
+        } else if (b == 0) {
+          d.rl[i] = li;
+          d.ru[i] = ui;
         }
-        if (finite_bound(li)) bn += li * li;
-        if (finite_bound(ui) && ui != li) bn += ui * ui;
-        if (cs.y0) y[static_cast<size_t>(i) * B + b] = cs.y0[i] / P.R[i] / P.gamma;
+        if (cases[q].y0) d.y[static_cast<size_t>(i) * B + b] = cases[q].y0[i] / P.R[i] / P.gamma;
       }
-      if (cn > 1e-20 && bn > 1e-20) omega[b] = std::sqrt(cn) / std::sqrt(bn);
     }
+    return d;
+  };
 
+  RunScalars sc;
+  sc.eta = eta;
+  sc.beta = P.beta;
+  sc.gamma = P.gamma;
+  sc.offset = P.offset;
+  sc.tol = o.tolerance;
+  auto make_run = [&](const std::vector<int>& ids, int B) {
     RunSetup rs;
     rs.m = m;
     rs.n = n;
@@ -177,107 +202,177 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       ms->xcd = o.xcd_remap == 1 || (o.xcd_remap < 0 && B >= 16);
     }
     auto run = DeviceRun::create(o.device, P, rs);
-    run->upload(c.data(), lo.data(), up.data(), rl.data(), ru.data(), x.data(), y.data());
-    RunScalars sc;
-    sc.eta = eta;
-    sc.beta = P.beta;
-    sc.gamma = P.gamma;
-    sc.offset = P.offset;
-    sc.tol = o.tolerance;
-    run->init_state(sc, omega.data(), cNorm.data(), rhsNorm.data(), pad.data());
+    LaneData d = lane_data(ids, B);
+    run->upload(d.c.data(), d.lo.data(), d.up.data(), d.rl.data(), d.ru.data(), d.x.data(), d.y.data());
+    run->init_state(sc, d.omega.data(), d.cNorm.data(), d.rhsNorm.data(), d.pad.data());
+    return run;
+  };
+
+  pdhg_stats acc{};
+  if (stats) {
+    acc.trace = stats->trace;
+    acc.trace_capacity = stats->trace_capacity;
+  }
+  acc.precision = o.precision;
+  long long maxItersRun = 0;
+  double solveSeconds = 0, kernelSeconds = 0, transfer = 0, setupSeconds = 0;
+  bool timed = false;
+
+  for (int first = 0; first < k; first += 64) {
+    const auto tChunk = Clock::now();
+    std::vector<int> ids;  // lane -> case
+    for (int q = first; q < std::min(k, first + 64); ++q) ids.push_back(q);
+    int B = next_pow2(static_cast<int>(ids.size()));
+    auto run = make_run(ids, B);
     transfer += since_s(tChunk);
     if (first == 0) setupSeconds = since_s(tSetup);
+    if (first == 0) {
+      double colB = 0, rowB = 0;
+      run->kernel_bytes(&colB, &rowB);
+      acc.iter_bytes = colB + rowB;
+      acc.col_kernel_bytes = colB;
+      acc.row_kernel_bytes = rowB;
+      const double vs = o.precision == PDHG_FP64 ? 8.0 : 4.0;
+      acc.spmv_bytes_per_iter = 2 * ((vs + 4.0) * P.A.nnz()) + 4.0 * (m + 1) + 4.0 * (n + 1) +
+                                2 * vs * static_cast<double>(B) * (m + n);
+      acc.batch = B;
+    }
 
     // ---- iterations (src/pdhg/pdhg.cpp:588-662), one check per block ----
     const auto tSolve = Clock::now();
-    run->start();
+    run->start(true);
     std::vector<CheckRecord> rec(B);
     std::vector<int> done(B), restarts(B);
     std::vector<long long> doneIter(B);
+    std::vector<char> extracted(k, 0);
     long long it = 0;
     int checks = 0;
     bool expired = false;
-    int remaining = kc;
+    auto lastProgress = Clock::now();
+
+    // Writes the results of lanes `which` of the current run (src/pdhg/pdhg.cpp:571-587).
+    auto extract = [&](const std::vector<int>& which, bool final) {
+      const auto tX = Clock::now();
+      std::vector<double> xs(static_cast<size_t>(n) * B), ys(static_cast<size_t>(m) * B);
+      run->download(0, xs.data());
+      run->download(1, ys.data());
+      transfer += since_s(tX);
+      for (int b : which) {
+        const int q = ids[b];
+        pdhg_result& r = results[q];
+        const CheckRecord& cr = rec[b];
+        if (done[b]) {
+          r.status = PDHG_OPTIMAL;
+          r.iterations = doneIter[b];
+        } else {
+          PDHG_REQUIRE(final, "internal: extracting a live instance");
+          r.status = expired ? PDHG_TIME_LIMIT : PDHG_ITERATION_LIMIT;
+          r.iterations = expired ? it - 1 : it;
+        }
+        r.restarts = restarts[b];
+        r.objective = cr.pobj;
+        r.dual_objective = cr.dobj;
+        r.rel_primal = cr.relP;
+        r.rel_dual = cr.relD;
+        r.rel_gap = cr.gap;
+        r.relative_kkt = std::max(cr.relP, std::max(cr.relD, cr.gap));
+        r.primal_weight = cr.omega;
+        if (r.x)
+          for (int j = 0; j < n; ++j) r.x[j] = xs[static_cast<size_t>(j) * B + b] * P.C[j] * P.beta;
+        if (r.row_dual)
+          for (int i = 0; i < m; ++i) r.row_dual[i] = ys[static_cast<size_t>(i) * B + b] * P.R[i] * P.gamma;
+        acc.restarts += restarts[b];
+        extracted[q] = 1;
+      }
+    };
+
     while (it < o.max_iterations) {
       const int nIt = static_cast<int>(std::min<long long>(o.check_every, o.max_iterations - it));
       run->run_block(nIt);
       run->fetch(rec.data(), done.data(), doneIter.data(), restarts.data());
       it += nIt;
       ++checks;
-      remaining = 0;
-      for (int b = 0; b < kc; ++b) remaining += done[b] ? 0 : 1;
-      if (first == 0 && acc.trace && acc.trace_length < acc.trace_capacity &&
-          static_cast<long long>(rec[0].it) == it - 1) {
-        std::memcpy(acc.trace + 8 * acc.trace_length, &rec[0], sizeof(CheckRecord));
-        acc.trace_length++;
+      const int real = static_cast<int>(ids.size());
+      int remaining = 0;
+      for (int b = 0; b < real; ++b) remaining += done[b] ? 0 : 1;
+      // Trace of the first case of the first chunk while it is live.
+      if (first == 0 && acc.trace && acc.trace_length < acc.trace_capacity) {
+        for (int b = 0; b < real; ++b)
+          if (ids[b] == 0 && static_cast<long long>(rec[b].it) == it - 1) {
+            std::memcpy(acc.trace + 8 * acc.trace_length, &rec[b], sizeof(CheckRecord));
+            acc.trace_length++;
+          }
       }
-      if (o.log_interval > 0 && kc == 1 && checks % o.log_interval == 0)
+      if (o.log_interval > 0 && k == 1 && checks % o.log_interval == 0)
         std::fprintf(stderr, "  pdhg %8lld  pobj % .8e  dobj % .8e  relP %.2e  relD %.2e  gap %.2e  w %.2e  restarts %d\n",
                      it, rec[0].pobj, rec[0].dobj, rec[0].relP, rec[0].relD, rec[0].gap, rec[0].omega, restarts[0]);
+      if (o.progress_seconds > 0 && since_s(lastProgress) >= o.progress_seconds) {
+        lastProgress = Clock::now();
+        std::fprintf(stderr, "  pdhg batch: iteration %lld, %d of %d live in a run of width %d, %.1f s\n", it,
+                     remaining, real, B, since_s(tSolve));
+      }
       if (remaining == 0) break;
       if (o.time_limit > 0 && since_s(tSetup) >= o.time_limit) {
         expired = true;
         break;
       }
+      // Compaction: finished instances leave; the live ones continue in a
+      // run of the next power of two, with their iterates, anchors and
+      // restart state unchanged.
+      if (o.compact && B > 1 && remaining <= B / 2) {
+        std::vector<int> fin, live;
+        for (int b = 0; b < real; ++b) (done[b] ? fin : live).push_back(b);
+        std::vector<int> newly;
+        for (int b : fin)
+          if (!extracted[ids[b]]) newly.push_back(b);
+        extract(newly, false);
+        LaneState ls;
+        run->export_lanes(live, ls);
+        kernelSeconds += run->stop();
+        acc.kernel_launches += run->launches;
+        acc.graph_launches += run->graphLaunches;
+        acc.bytes_uploaded += run->bytesUp;
+        acc.bytes_downloaded += run->bytesDown;
+        std::vector<int> nids;
+        for (int b : live) nids.push_back(ids[b]);
+        ids = nids;
+        B = next_pow2(static_cast<int>(ids.size()));
+        run = make_run(ids, B);
+        run->import_lanes(ls);
+        run->start(false);
+        rec.assign(B, CheckRecord{});
+        done.assign(B, 0);
+        restarts.assign(B, 0);
+        doneIter.assign(B, -1);
+      }
     }
     kernelSeconds += run->stop();
-    if (o.kernel_timing_reps > 0 && first == 0) {
-      // After the loop: every instance is released so each launch does the
-      // full work of an iteration (the results were already reached).
-      run->time_kernels(o.kernel_timing_reps, &acc.col_kernel_ms, &acc.row_kernel_ms);
-    }
     maxItersRun = std::max(maxItersRun, it);
-
-    // ---- extraction (src/pdhg/pdhg.cpp:571-587, 668-680) ----
-    const auto tX = Clock::now();
-    std::vector<double> xs(nB), ys(mB);
-    run->download(0, xs.data());
-    run->download(1, ys.data());
-    transfer += since_s(tX);
-    for (int b = 0; b < kc; ++b) {
-      pdhg_result& r = results[first + b];
-      const CheckRecord& q = rec[b];
-      if (done[b]) {
-        r.status = PDHG_OPTIMAL;
-        r.iterations = doneIter[b];
-      } else {
-        r.status = expired ? PDHG_TIME_LIMIT : PDHG_ITERATION_LIMIT;
-        r.iterations = expired ? it - 1 : it;
-      }
-      r.restarts = restarts[b];
-      r.objective = q.pobj;
-      r.dual_objective = q.dobj;
-      r.rel_primal = q.relP;
-      r.rel_dual = q.relD;
-      r.rel_gap = q.gap;
-      r.relative_kkt = std::max(q.relP, std::max(q.relD, q.gap));
-      r.primal_weight = q.omega;
-      if (r.x)
-        for (int j = 0; j < n; ++j) r.x[j] = xs[static_cast<size_t>(j) * B + b] * P.C[j] * P.beta;
-      if (r.row_dual)
-        for (int i = 0; i < m; ++i) r.row_dual[i] = ys[static_cast<size_t>(i) * B + b] * P.R[i] * P.gamma;
-      acc.restarts += restarts[b];
+    std::vector<int> rest;
+    for (int b = 0; b < static_cast<int>(ids.size()); ++b)
+      if (!extracted[ids[b]]) rest.push_back(b);
+    extract(rest, true);
+    if (o.kernel_timing_reps > 0 && !timed) {
+      // After the results were read, every instance is released so each
+      // timed launch does the full work of an iteration.
+      run->time_kernels(o.kernel_timing_reps, &acc.col_kernel_ms, &acc.row_kernel_ms);
+      double colB = 0, rowB = 0;
+      run->kernel_bytes(&colB, &rowB);
+      acc.col_kernel_bytes = colB;
+      acc.row_kernel_bytes = rowB;
+      timed = true;
     }
     solveSeconds += since_s(tSolve);
     acc.kernel_launches += run->launches;
     acc.graph_launches += run->graphLaunches;
     acc.bytes_uploaded += run->bytesUp;
     acc.bytes_downloaded += run->bytesDown;
-    double colB = 0, rowB = 0;
-    run->kernel_bytes(&colB, &rowB);
-    acc.iter_bytes = colB + rowB;
-    acc.col_kernel_bytes = colB;
-    acc.row_kernel_bytes = rowB;
-    const double vs = o.precision == PDHG_FP64 ? 8.0 : 4.0;
-    acc.spmv_bytes_per_iter = 2 * ((vs + 4.0) * P.A.nnz()) + 4.0 * (m + 1) + 4.0 * (n + 1) +
-                              2 * vs * static_cast<double>(B) * (m + n);
     if (k == 1) {
       acc.primal_residual = rec[0].relP;
       acc.dual_residual = rec[0].relD;
       acc.gap = rec[0].gap;
       acc.primal_weight = rec[0].omega;
     }
-    acc.batch = B;
   }
   if (stats) {
     acc.iterations = maxItersRun;
@@ -290,7 +385,6 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     std::snprintf(acc.device_name, sizeof(acc.device_name), "%s", di.name.c_str());
     *stats = acc;
   }
-  (void)chunks;
 }
 
 }  // namespace pdhgamd

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pdhg_hip.h"
@@ -115,6 +116,64 @@ __device__ inline Acc row_dot(const int* __restrict__ idx, const V* __restrict__
   return (a0 + a1) + (a2 + a3);
 }
 
+// B = 64: a whole wavefront serves one row (lane = instance), so the row's
+// bounds are wave-uniform.  Column indices and values then come through the
+// scalar cache (s_load) sixteen at a time, and the sixteen 64-lane gathers of
+// a chunk are issued back to back before the first multiply-add waits.  The
+// last partial chunk is predicated: its spare slots re-read the row's first
+// column (a cache hit) with a zero weight, so no serial tail remains.
+template <typename Acc, typename V, typename X>
+__device__ inline Acc row_dot_wave(const int* __restrict__ idx, const V* __restrict__ val, const X* __restrict__ x,
+                                   int e0, int e1, int b) {
+  constexpr int CH = 16;
+  e0 = __builtin_amdgcn_readfirstlane(e0);
+  e1 = __builtin_amdgcn_readfirstlane(e1);
+  Acc a[4] = {0, 0, 0, 0};
+  int e = e0;
+  for (; e + CH <= e1; e += CH) {
+    int c[CH];
+    Acc v[CH], g[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      c[u] = idx[e + u];
+      v[u] = static_cast<Acc>(val[e + u]);
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) g[u] = static_cast<Acc>(x[(static_cast<size_t>(c[u]) << 6) + b]);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) a[u & 3] += v[u] * g[u];
+  }
+  if (e < e1) {
+    const int last = e1 - 1;
+    int c[CH];
+    Acc v[CH], g[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int p = min(e + u, last);  // always a valid entry of this row
+      c[u] = idx[p];
+      const Acc w = static_cast<Acc>(val[p]);
+      v[u] = e + u <= last ? w : static_cast<Acc>(0);
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) g[u] = static_cast<Acc>(x[(static_cast<size_t>(c[u]) << 6) + b]);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) a[u & 3] += v[u] * g[u];
+  }
+  return (a[0] + a[1]) + (a[2] + a[3]);
+}
+
+// Row dot product for the lane layout of LOGB (the B = 64 case uses the
+// wave-uniform path above; there the plan always has one lane per instance).
+template <int LOGB, typename Acc, typename V, typename X>
+__device__ inline Acc row_sum(const int* __restrict__ ptr, const int* __restrict__ idx, const V* __restrict__ val,
+                              const X* __restrict__ x, const Slot& s, int b) {
+  if constexpr (LOGB == 6) {
+    return row_dot_wave<Acc>(idx, val, x, ptr[s.row], ptr[s.row + 1], b);
+  } else {
+    return row_dot<LOGB, Acc>(idx, val, x, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
+  }
+}
+
 // Per-workgroup FP64 partial sums of NQ quantities for each instance
 // b = threadIdx % B, stored at out[((q0 + q) * B + b) * nblk + blk].
 template <int LOGB, int NQ>
@@ -185,7 +244,7 @@ __global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict
   const bool rst = st.applyAt[b] == it;
   const T* __restrict__ ysrc = rst ? yp : y;
   T acc = 0;
-  if (s.row >= 0) acc = row_dot<LOGB, T>(idx, val, ysrc, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
+  if (s.row >= 0) acc = row_sum<LOGB, T>(ptr, idx, val, ysrc, s, b);
   acc = reduce_sub<LOGB>(acc, s.logS);
   double q[2] = {0.0, 0.0};
   if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
@@ -238,9 +297,8 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
   T acc = 0;
   double acc64 = 0;
   if (s.row >= 0) {
-    const int e0 = ptr[s.row] + s.sub, e1 = ptr[s.row + 1], step = 1 << s.logS;
-    acc = row_dot<LOGB, T>(idx, val, xp, e0, e1, step, b);
-    if (CHECK) acc64 = row_dot<LOGB, double>(idx, val64, xp, e0, e1, step, b);
+    acc = row_sum<LOGB, T>(ptr, idx, val, xp, s, b);
+    if (CHECK) acc64 = row_sum<LOGB, double>(ptr, idx, val64, xp, s, b);
   }
   acc = reduce_sub<LOGB>(acc, s.logS);
   if (CHECK) acc64 = reduce_sub<LOGB>(acc64, s.logS);
@@ -311,7 +369,7 @@ __global__ __launch_bounds__(kWG) void k_colcheck(RowPlan plan, const int* __res
   const Slot s = locate<LOGB>(plan, bid);
   const int b = threadIdx.x & (B - 1);
   double acc = 0;
-  if (s.row >= 0) acc = row_dot<LOGB, double>(idx, val64, yp, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
+  if (s.row >= 0) acc = row_sum<LOGB, double>(ptr, idx, val64, yp, s, b);
   acc = reduce_sub<LOGB>(acc, s.logS);
   double q[3] = {0.0, 0.0, 0.0};
   if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
@@ -434,7 +492,7 @@ __global__ __launch_bounds__(kWG) void k_spmv(RowPlan plan, const int* __restric
   const Slot s = locate<LOGB>(plan, bid);
   const int b = threadIdx.x & (B - 1);
   T acc = 0;
-  if (s.row >= 0) acc = row_dot<LOGB, T>(idx, val, in, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
+  if (s.row >= 0) acc = row_sum<LOGB, T>(ptr, idx, val, in, s, b);
   acc = reduce_sub<LOGB>(acc, s.logS);
   if (s.row >= 0 && s.sub == 0) out[(static_cast<size_t>(s.row) << LOGB) + b] = acc;
 }
@@ -884,10 +942,12 @@ class RunImpl final : public DeviceRun {
     HIP_OK(hipStreamSynchronize(stream_));
   }
 
-  void start() override {
-    launch_spmv(false, x_, ax_);
-    HIP_OK(hipMemcpyAsync(ax0_, ax_, sizeof(T) * m_ * static_cast<size_t>(B_), hipMemcpyDeviceToDevice, stream_));
-    HIP_OK(hipMemcpyAsync(axp_, ax_, sizeof(T) * m_ * static_cast<size_t>(B_), hipMemcpyDeviceToDevice, stream_));
+  void start(bool initAx) override {
+    if (initAx) {
+      launch_spmv(false, x_, ax_);
+      HIP_OK(hipMemcpyAsync(ax0_, ax_, sizeof(T) * m_ * static_cast<size_t>(B_), hipMemcpyDeviceToDevice, stream_));
+      HIP_OK(hipMemcpyAsync(axp_, ax_, sizeof(T) * m_ * static_cast<size_t>(B_), hipMemcpyDeviceToDevice, stream_));
+    }
     HIP_OK(hipEventRecord(ev0_, stream_));
   }
 
@@ -971,6 +1031,89 @@ class RunImpl final : public DeviceRun {
     const double nc = static_cast<double>(nc_), nb = static_cast<double>(nbnd_), mr = static_cast<double>(mr_);
     *colBytes = mat + 4.0 * (n_ + 1) + vs * mB + vs * (2 * nB /*x, x0*/ + nc + 2 * nb) + vs * 2 * nB /*x, xp*/;
     *rowBytes = mat + 4.0 * (m_ + 1) + vs * nB + vs * (4 * mB /*y, y0, ax, ax0*/ + 2 * mr) + vs * 2 * mB /*y, ax*/;
+  }
+
+  void export_lanes(const std::vector<int>& lanes, LaneState& o) override {
+    const int k = static_cast<int>(lanes.size());
+    o.count = k;
+    auto pick = [&](const T* d, int rows, std::vector<double>& out) {
+      std::vector<T> full(static_cast<size_t>(rows) * B_);
+      HIP_OK(hipMemcpyAsync(full.data(), d, full.size() * sizeof(T), hipMemcpyDeviceToHost, stream_));
+      HIP_OK(hipStreamSynchronize(stream_));
+      bytesDown += full.size() * sizeof(T);
+      out.resize(static_cast<size_t>(rows) * k);
+      for (int i = 0; i < rows; ++i)
+        for (int q = 0; q < k; ++q)
+          out[static_cast<size_t>(i) * k + q] = full[static_cast<size_t>(i) * B_ + lanes[q]];
+    };
+    pick(x_, n_, o.x);
+    pick(x0_, n_, o.x0);
+    pick(xp_, n_, o.xp);
+    pick(y_, m_, o.y);
+    pick(y0_, m_, o.y0);
+    pick(yp_, m_, o.yp);
+    pick(ax_, m_, o.ax);
+    pick(ax0_, m_, o.ax0);
+    pick(axp_, m_, o.axp);
+    auto sel = [&](const auto* d, auto& out) {
+      using U = std::remove_const_t<std::remove_pointer_t<decltype(d)>>;
+      std::vector<U> full(B_);
+      HIP_OK(hipMemcpyAsync(full.data(), d, B_ * sizeof(U), hipMemcpyDeviceToHost, stream_));
+      HIP_OK(hipStreamSynchronize(stream_));
+      out.resize(k);
+      for (int q = 0; q < k; ++q) out[q] = full[lanes[q]];
+    };
+    sel(st_.tau, o.tau);
+    sel(st_.sigma, o.sigma);
+    sel(st_.omega, o.omega);
+    sel(st_.fpR, o.fpR);
+    sel(st_.fpPrev, o.fpPrev);
+    sel(st_.iter, o.iter);
+    sel(st_.since, o.since);
+    sel(st_.restartAt, o.restartAt);
+    sel(st_.applyAt, o.applyAt);
+    sel(st_.restarts, o.restarts);
+  }
+
+  void import_lanes(const LaneState& in) override {
+    const int k = in.count;
+    PDHG_REQUIRE(k <= B_, "import: more lanes than the run holds");
+    auto place = [&](T* d, int rows, const std::vector<double>& src) {
+      std::vector<T> full(static_cast<size_t>(rows) * B_);
+      HIP_OK(hipMemcpyAsync(full.data(), d, full.size() * sizeof(T), hipMemcpyDeviceToHost, stream_));
+      HIP_OK(hipStreamSynchronize(stream_));
+      for (int i = 0; i < rows; ++i)
+        for (int q = 0; q < k; ++q)
+          full[static_cast<size_t>(i) * B_ + q] = static_cast<T>(src[static_cast<size_t>(i) * k + q]);
+      copy_h2d(d, full.data(), full.size());
+    };
+    place(x_, n_, in.x);
+    place(x0_, n_, in.x0);
+    place(xp_, n_, in.xp);
+    place(y_, m_, in.y);
+    place(y0_, m_, in.y0);
+    place(yp_, m_, in.yp);
+    place(ax_, m_, in.ax);
+    place(ax0_, m_, in.ax0);
+    place(axp_, m_, in.axp);
+    auto put_state = [&](auto* d, const auto& src) {
+      using U = std::remove_pointer_t<decltype(d)>;
+      std::vector<U> full(B_);
+      HIP_OK(hipMemcpyAsync(full.data(), d, B_ * sizeof(U), hipMemcpyDeviceToHost, stream_));
+      HIP_OK(hipStreamSynchronize(stream_));
+      for (int q = 0; q < k; ++q) full[q] = static_cast<U>(src[q]);
+      copy_h2d(d, full.data(), B_);
+    };
+    put_state(st_.tau, in.tau);
+    put_state(st_.sigma, in.sigma);
+    put_state(st_.omega, in.omega);
+    put_state(st_.fpR, in.fpR);
+    put_state(st_.fpPrev, in.fpPrev);
+    put_state(st_.iter, in.iter);
+    put_state(st_.since, in.since);
+    put_state(st_.restartAt, in.restartAt);
+    put_state(st_.applyAt, in.applyAt);
+    put_state(st_.restarts, in.restarts);
   }
 
  private:

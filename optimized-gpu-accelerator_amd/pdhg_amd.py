@@ -45,7 +45,7 @@ class Options(C.Structure):
                 ("rescale", C.c_int), ("reflection", C.c_double), ("log_interval", C.c_int),
                 ("precision", C.c_int), ("use_graph", C.c_int), ("lanes_per_row", C.c_int), ("row_map", C.c_int),
                 ("xcd_remap", C.c_int), ("time_limit", C.c_double), ("device", C.c_int),
-                ("kernel_timing_reps", C.c_int)]
+                ("kernel_timing_reps", C.c_int), ("compact", C.c_int), ("progress_seconds", C.c_double)]
 
 
 class Stats(C.Structure):
