@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-iterations", type=int, default=1_000_000,
                     help="cap per instance (the reference default, src/pdhg/pdhg.h:34); unconverged = unsolved")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run one device copy of MIB MiB (units of the HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched family shards by "
@@ -101,6 +103,18 @@ def main():
     info = P.gpu_info()
     if not info["available"]:
         raise SystemExit(f"no usable gfx950 device: {info}")
+    if args.copy_calibration:
+        # A plain device-to-device copy of a known size: its FETCH_SIZE /
+        # WRITE_SIZE counter reading calibrates the units of the HBM-traffic
+        # counters collected for the iteration kernels in the same process.
+        import torch
+
+        src = torch.ones(args.copy_calibration << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_calibration} MiB device to device")
+        del src, dst
     prec = P.FP64 if args.precision == "fp64" else P.FP32
     lp = P.gen_refinery(SEED_LP)
     thetas, costs = P.gen_family(lp, SEED_THETA, K, 0.3)
