@@ -107,14 +107,19 @@ def main():
         # A plain device-to-device copy of a known size: its FETCH_SIZE /
         # WRITE_SIZE counter reading calibrates the units of the HBM-traffic
         # counters collected for the iteration kernels in the same process.
-        import torch
+        import ctypes
 
-        src = torch.ones(args.copy_calibration << 20, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        hip = ctypes.CDLL("libamdhip64.so")
+        size = args.copy_calibration << 20
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(size)) == 0
+        assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(size)) == 0
+        assert hip.hipMemset(src, 1, ctypes.c_size_t(size)) == 0
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(size), 3) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipDeviceSynchronize() == 0
+        hip.hipFree(src)
+        hip.hipFree(dst)
         log(f"calibration copy: {args.copy_calibration} MiB device to device")
-        del src, dst
     prec = P.FP64 if args.precision == "fp64" else P.FP32
     lp = P.gen_refinery(SEED_LP)
     thetas, costs = P.gen_family(lp, SEED_THETA, K, 0.3)

@@ -118,14 +118,14 @@ __device__ inline Acc row_dot(const int* __restrict__ idx, const V* __restrict__
 
 // B = 64: a whole wavefront serves one row (lane = instance), so the row's
 // bounds are wave-uniform.  Column indices and values then come through the
-// scalar cache (s_load) sixteen at a time, and the sixteen 64-lane gathers of
+// scalar cache (s_load) eight at a time, and the eight 64-lane gathers of
 // a chunk are issued back to back before the first multiply-add waits.  The
 // last partial chunk is predicated: its spare slots re-read the row's first
 // column (a cache hit) with a zero weight, so no serial tail remains.
 template <typename Acc, typename V, typename X>
 __device__ inline Acc row_dot_wave(const int* __restrict__ idx, const V* __restrict__ val, const X* __restrict__ x,
                                    int e0, int e1, int b) {
-  constexpr int CH = 16;
+  constexpr int CH = 8;
   e0 = __builtin_amdgcn_readfirstlane(e0);
   e1 = __builtin_amdgcn_readfirstlane(e1);
   Acc a[4] = {0, 0, 0, 0};
