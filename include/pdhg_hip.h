@@ -48,7 +48,10 @@ enum {
 
 /* Iterate precision. */
 enum { PDHG_FP32 = 0, /* FP32 iterates + SpMV, FP64 norms/restarts/KKT */
-       PDHG_FP64 = 1  /* FP64 everywhere (the reference's arithmetic)   */ };
+       PDHG_FP64 = 1, /* FP64 everywhere (the reference's arithmetic)   */
+       PDHG_FP32_AUTO = 2 /* FP32 iterates; the run moves to FP64 iterates when
+                             half its live instances stop improving (the FP32
+                             resolution floor) */ };
 
 /* Read-only LP description (minimization form), CSC matrix. */
 typedef struct pdhg_lp {
@@ -89,7 +92,7 @@ typedef struct pdhg_options {
   int rescale;                 /* objective/bound rescaling beta, gamma (1) */
   double reflection;           /* rho (1.0) */
   int log_interval;            /* log every N checks to stderr (0 = off) */
-  int precision;               /* PDHG_FP32 (default) or PDHG_FP64 */
+  int precision;               /* PDHG_FP32 (default), PDHG_FP64 or PDHG_FP32_AUTO */
   int use_graph;               /* capture check_every iterations in a hipGraph (1) */
   int lanes_per_row;           /* 0 = autotune; else nnz-per-lane target (1..64) */
   int row_map;                 /* 0 = auto, 1 = row-length buckets, 2 = one mapping */
@@ -113,6 +116,7 @@ typedef struct pdhg_stats {
   double spectral_norm;
   int batch;                   /* instances in the run */
   int precision;
+  long long fp64_switch_iteration; /* PDHG_FP32_AUTO: iteration of the move to FP64 (-1: none) */
   double spmv_bytes_per_iter;  /* algorithmic bytes of the 2 SpMVs per iteration */
   double iter_bytes;           /* algorithmic bytes of one fused iteration */
   double col_kernel_ms, row_kernel_ms;        /* mean launch time (kernel_timing_reps > 0) */

@@ -184,12 +184,15 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
   sc.gamma = P.gamma;
   sc.offset = P.offset;
   sc.tol = o.tolerance;
+  int runPrec = o.precision == PDHG_FP64 ? PDHG_FP64 : PDHG_FP32;
+  std::vector<double> refKkt(k, kInf);
+  std::vector<int> lastGain(k, 0);
   auto make_run = [&](const std::vector<int>& ids, int B) {
     RunSetup rs;
     rs.m = m;
     rs.n = n;
     rs.B = B;
-    rs.precision = o.precision;
+    rs.precision = runPrec == PDHG_FP64 ? PDHG_FP64 : PDHG_FP32;
     rs.sharedCost = shC;
     rs.sharedColBounds = shB;
     rs.sharedRowBounds = shR;
@@ -214,6 +217,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     acc.trace_capacity = stats->trace_capacity;
   }
   acc.precision = o.precision;
+  acc.fp64_switch_iteration = -1;
   long long maxItersRun = 0;
   double solveSeconds = 0, kernelSeconds = 0, transfer = 0, setupSeconds = 0;
   bool timed = false;
@@ -316,10 +320,29 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
         expired = true;
         break;
       }
-      // Compaction: finished instances leave; the live ones continue in a
-      // run of the next power of two, with their iterates, anchors and
-      // restart state unchanged.
-      if (o.compact && B > 1 && remaining <= B / 2) {
+      // FP32 resolution floor: an instance whose best relative KKT has not
+      // halved over the last 256 checks is stalled; once half the live
+      // instances are, the run moves to FP64 iterates.
+      bool toFp64 = false;
+      if (o.precision == PDHG_FP32_AUTO && runPrec == PDHG_FP32) {
+        int stalled = 0;
+        for (int b = 0; b < real; ++b) {
+          if (done[b]) continue;
+          const int q = ids[b];
+          const double kkt = std::max(rec[b].relP, std::max(rec[b].relD, rec[b].gap));
+          if (kkt < 0.5 * refKkt[q]) {
+            refKkt[q] = kkt;
+            lastGain[q] = checks;
+          }
+          if (checks - lastGain[q] >= 256) ++stalled;
+        }
+        toFp64 = stalled > 0 && 2 * stalled >= remaining;
+      }
+      // Re-pack: finished instances leave (compaction) and/or the run changes
+      // precision; live instances keep their iterates, anchors and restart
+      // state, moved through an FP64 host copy.
+      const bool compactNow = o.compact && B > 1 && remaining <= B / 2;
+      if (compactNow || toFp64) {
         std::vector<int> fin, live;
         for (int b = 0; b < real; ++b) (done[b] ? fin : live).push_back(b);
         std::vector<int> newly;
@@ -337,6 +360,12 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
         for (int b : live) nids.push_back(ids[b]);
         ids = nids;
         B = next_pow2(static_cast<int>(ids.size()));
+        if (toFp64) {
+          runPrec = PDHG_FP64;
+          if (acc.fp64_switch_iteration < 0) acc.fp64_switch_iteration = it;
+          if (o.progress_seconds > 0)
+            std::fprintf(stderr, "  pdhg batch: iteration %lld, FP32 progress stalled, continuing in FP64\n", it);
+        }
         run = make_run(ids, B);
         run->import_lanes(ls);
         run->start(false);

@@ -41,8 +41,12 @@ namespace pdhgamd {
 
 namespace {
 
-constexpr int kWG = 256;       // threads per workgroup (4 wave64)
-constexpr int kLogWG = 8;
+#ifndef PDHG_LOG_WG
+#define PDHG_LOG_WG 8
+#endif
+constexpr int kLogWG = PDHG_LOG_WG;  // threads per workgroup: 256 (4 wave64)
+constexpr int kWG = 1 << kLogWG;
+constexpr int kWaves = kWG / 64;
 constexpr int kMaxBuckets = 8;
 
 // ---------------------------------------------------------------------------
@@ -191,8 +195,11 @@ __device__ inline void block_partials(const double (&v)[NQ], double* __restrict_
   if (threadIdx.x < B) {
     const int b = threadIdx.x;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
-      out[(static_cast<size_t>(q0 + q) * B + b) * nblk + blk] = ((sh[q][0][b] + sh[q][1][b]) + sh[q][2][b]) + sh[q][3][b];
+    for (int q = 0; q < NQ; ++q) {
+      double t = sh[q][0][b];
+      for (int w = 1; w < kWaves; ++w) t += sh[q][w][b];
+      out[(static_cast<size_t>(q0 + q) * B + b) * nblk + blk] = t;
+    }
   }
 }
 
@@ -425,7 +432,10 @@ __global__ __launch_bounds__(kWG) void k_finalize(Inst st, const double* __restr
   __syncthreads();
   if (threadIdx.x != 0) return;
   double S[NQ];
-  for (int q = 0; q < NQ; ++q) S[q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+  for (int q = 0; q < NQ; ++q) {
+    S[q] = sh[q][0];
+    for (int w = 1; w < kWaves; ++w) S[q] += sh[q][w];
+  }
   const double dx2 = S[0], ax2 = S[1], dres2 = S[2], pobjS = S[3], dobjC = S[4];
   const double pres2 = S[5], dobjR = S[6], dy2 = S[7], ay2 = S[8];
   const long long it = st.iter[b] + blockIters - 1;
@@ -559,7 +569,11 @@ __device__ inline void op_block_sum3(double a, double bq, double cq, double* out
   }
   __syncthreads();
   if (threadIdx.x == 0)
-    for (int q = 0; q < 3; ++q) out[q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+    for (int q = 0; q < 3; ++q) {
+      double t = sh[q][0];
+      for (int w = 1; w < kWaves; ++w) t += sh[q][w];
+      out[q] = t;
+    }
 }
 
 template <typename T>

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpdhg_hip.so")
 
 OPTIMAL, TIME_LIMIT, ITERATION_LIMIT = 1, 5, 6
-FP32, FP64 = 0, 1
+FP32, FP64, FP32_AUTO = 0, 1, 2
 (V_X, V_XP, V_X0, V_ATY, V_ATYP, V_C, V_LO, V_UP,
  V_Y, V_YP, V_Y0, V_AX, V_AXP, V_AX0, V_RL, V_RU) = range(16)
 
@@ -55,7 +55,8 @@ class Stats(C.Structure):
                 ("transfer_seconds", C.c_double), ("bytes_uploaded", C.c_longlong),
                 ("bytes_downloaded", C.c_longlong), ("kernel_launches", C.c_longlong),
                 ("graph_launches", C.c_longlong), ("spectral_norm", C.c_double), ("batch", C.c_int),
-                ("precision", C.c_int), ("spmv_bytes_per_iter", C.c_double), ("iter_bytes", C.c_double),
+                ("precision", C.c_int), ("fp64_switch_iteration", C.c_longlong),
+                ("spmv_bytes_per_iter", C.c_double), ("iter_bytes", C.c_double),
                 ("col_kernel_ms", C.c_double), ("row_kernel_ms", C.c_double), ("col_kernel_bytes", C.c_double),
                 ("row_kernel_bytes", C.c_double),
                 ("trace", _dp), ("trace_capacity", C.c_int), ("trace_length", C.c_int),
