@@ -212,6 +212,33 @@ int pdhg_certify(const pdhg_lp* lp, const double* x, const double* y, double obj
 int pdhg_certify_batch(const pdhg_lp* base, const pdhg_case* cases, int k, const double* xs, const double* ys,
                        const double* objectives, int threads, double* out);
 
+/* ---- learned warm start (no reference counterpart: the reference hard-wires
+ * x0 = clamp(0), y0 = 0, src/pdhg/pdhg.cpp:509,519) ----------------------- */
+/* POD basis of k solution snapshots Z (k x d, row-major), host FP64 with a
+ * cyclic Jacobi eigensolver on the snapshot Gram matrix.  Writes mean[d],
+ * basis[d x rank] (row-major), coeffs[k x rank], energy[rank]; returns the
+ * number of modes kept (>= 1) or -1. */
+int pdhg_pod_basis(const double* Z, int k, int d, int rank, double* mean, double* basis, double* coeffs,
+                   double* energy);
+/* out (N x d) = mean + coeffs (N x rank) basis^T on BF16 MFMA tiles with FP32
+ * accumulation (device 0). */
+int pdhg_decode_bf16(const double* basis, const double* mean, int d, int rank, const double* coeffs, int N,
+                     double* out);
+/* MLP in -> hidden (GELU) -> hidden (GELU) -> out; forward, backward and Adam
+ * are HIP kernels on BF16 MFMA tiles with FP32 accumulation and FP32 master
+ * weights.  Parameters: W1[hidden x in], b1, W2[hidden x hidden], b2,
+ * W3[out x hidden], b3 in one vector.  Loss: mean squared error. */
+typedef struct pdhg_mlp pdhg_mlp;
+pdhg_mlp* pdhg_mlp_create(int in, int hidden, int out, uint64_t seed, int device);
+void pdhg_mlp_destroy(pdhg_mlp* mlp);
+int pdhg_mlp_param_count(const pdhg_mlp* mlp);
+int pdhg_mlp_get_params(pdhg_mlp* mlp, double* params);
+int pdhg_mlp_set_params(pdhg_mlp* mlp, const double* params);
+int pdhg_mlp_forward(pdhg_mlp* mlp, const double* X, int N, double* Y);
+int pdhg_mlp_loss_grad(pdhg_mlp* mlp, const double* X, const double* T, int N, double* loss, double* grad);
+/* `steps` full-batch Adam steps (beta 0.9 / 0.999); losses[steps] may be NULL. */
+int pdhg_mlp_train(pdhg_mlp* mlp, const double* X, const double* T, int N, int steps, double lr, double* losses);
+
 /* ---- synthetic LP generators (owned models) ---------------------------- */
 typedef struct pdhg_model pdhg_model;
 /* Random LP with a known optimum: x* half zeros / half U(0,1), s*

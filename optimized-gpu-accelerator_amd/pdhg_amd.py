@@ -81,6 +81,8 @@ EXPORTS = [
     "pdhg_safe_dual_bound", "pdhg_certify", "pdhg_certify_batch",
     "pdhg_gen_random_lp", "pdhg_gen_refinery", "pdhg_gen_family", "pdhg_family_costs",
     "pdhg_gen_calibration", "pdhg_model_view", "pdhg_model_free",
+    "pdhg_pod_basis", "pdhg_decode_bf16", "pdhg_mlp_create", "pdhg_mlp_destroy", "pdhg_mlp_param_count",
+    "pdhg_mlp_get_params", "pdhg_mlp_set_params", "pdhg_mlp_forward", "pdhg_mlp_loss_grad", "pdhg_mlp_train",
 ]
 
 _lib = None
@@ -129,6 +131,17 @@ def lib() -> C.CDLL:
         L.pdhg_backend_kkt.argtypes = [C.c_void_p, _dp, _dp]
         L.pdhg_backend_diff2.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
         L.pdhg_backend_time_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.pdhg_pod_basis.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.pdhg_decode_bf16.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, C.c_int, _dp]
+        L.pdhg_mlp_create.restype = C.c_void_p
+        L.pdhg_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int]
+        L.pdhg_mlp_destroy.argtypes = [C.c_void_p]
+        L.pdhg_mlp_param_count.argtypes = [C.c_void_p]
+        L.pdhg_mlp_get_params.argtypes = [C.c_void_p, _dp]
+        L.pdhg_mlp_set_params.argtypes = [C.c_void_p, _dp]
+        L.pdhg_mlp_forward.argtypes = [C.c_void_p, _dp, C.c_int, _dp]
+        L.pdhg_mlp_loss_grad.argtypes = [C.c_void_p, _dp, _dp, C.c_int, _dp, _dp]
+        L.pdhg_mlp_train.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_int, C.c_double, _dp]
         _lib = L
     return _lib
 
@@ -468,3 +481,79 @@ class Backend:
         ms = C.c_double()
         _check(lib().pdhg_backend_time_spmv(self.h, int(transpose), reps, C.byref(ms)), "time_spmv")
         return ms.value
+
+
+# ---------------------------------------------------------------------------
+# Learned warm start: POD basis, BF16-MFMA decode and the MLP
+# ---------------------------------------------------------------------------
+def pod_basis(Z: np.ndarray, rank: int):
+    """POD of snapshots Z (k x d): returns (mean d, basis d x rank, coeffs k x rank, energy rank, used)."""
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    k, d = Z.shape
+    mean, basis = np.zeros(d), np.zeros((d, rank))
+    coeffs, energy = np.zeros((k, rank)), np.zeros(rank)
+    used = lib().pdhg_pod_basis(_d(Z), k, d, rank, _d(mean), _d(basis), _d(coeffs), _d(energy))
+    if used < 0:
+        raise RuntimeError(f"pod_basis: {_err()}")
+    return mean, basis, coeffs, energy, used
+
+
+def decode_bf16(basis: np.ndarray, mean: np.ndarray, coeffs: np.ndarray) -> np.ndarray:
+    """mean + coeffs @ basis.T on BF16 MFMA tiles (FP32 accumulation)."""
+    basis = np.ascontiguousarray(basis, dtype=np.float64)
+    mean = np.ascontiguousarray(mean, dtype=np.float64)
+    coeffs = np.ascontiguousarray(np.atleast_2d(coeffs), dtype=np.float64)
+    d, r = basis.shape
+    out = np.zeros((coeffs.shape[0], d))
+    _check(lib().pdhg_decode_bf16(_d(basis), _d(mean), d, r, _d(coeffs), coeffs.shape[0], _d(out)), "decode")
+    return out
+
+
+class Mlp:
+    """in -> hidden (GELU) -> hidden (GELU) -> out, trained on the GPU (BF16 MFMA, FP32 accumulate)."""
+
+    def __init__(self, n_in=4, hidden=128, n_out=32, seed=0, device=0):
+        self.shape = (n_in, hidden, n_out)
+        self.h = lib().pdhg_mlp_create(n_in, hidden, n_out, seed, device)
+        if not self.h:
+            raise RuntimeError(f"mlp_create: {_err()}")
+        self.n_params = lib().pdhg_mlp_param_count(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pdhg_mlp_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params)
+        _check(lib().pdhg_mlp_get_params(self.h, _d(p)), "mlp_get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        assert p.size == self.n_params
+        _check(lib().pdhg_mlp_set_params(self.h, _d(p)), "mlp_set_params")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        Y = np.zeros((X.shape[0], self.shape[2]))
+        _check(lib().pdhg_mlp_forward(self.h, _d(X), X.shape[0], _d(Y)), "mlp_forward")
+        return Y
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        loss, g = C.c_double(), np.zeros(self.n_params)
+        _check(lib().pdhg_mlp_loss_grad(self.h, _d(X), _d(T), X.shape[0], C.byref(loss), _d(g)), "mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, X, T, steps=2000, lr=1e-3):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        losses = np.zeros(steps)
+        _check(lib().pdhg_mlp_train(self.h, _d(X), _d(T), X.shape[0], steps, lr, _d(losses)), "mlp_train")
+        return losses

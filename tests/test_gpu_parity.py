@@ -129,3 +129,44 @@ def test_afiro_batch_equals_individual(gpu):
             print(prec, k, res[k].status, res[k].iterations, res[k].objective, ref[k]["objective"])
             assert res[k].status == P.OPTIMAL
             assert abs(res[k].objective - ref[k]["objective"]) <= 1e-5 * (1 + abs(ref[k]["objective"]))
+
+
+def small_refinery():
+    return P.gen_refinery(3, periods=2, streams=12, crudes=4, products=4, qualities=5, decades=3.0)
+
+
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32_AUTO])
+def test_refinery_family_matches_oracle(gpu, prec):
+    """Badly scaled blending family (10^U(-3,3) rows/columns), 8 price cases."""
+    lp = small_refinery()
+    _, costs = P.gen_family(lp, 4, 8, 0.3)
+    ref, _, ys_o, _ = O.oracle_solve(lp, costs, tol=1e-6, max_iter=400_000)
+    res, st, _ = P.solve_batch(lp, costs, P.default_options(tolerance=1e-6, precision=prec, max_iterations=400_000))
+    # Certified dual bounds of the engine's duals agree with those of the oracle's duals.
+    cert_o = P.certify_batch(lp, costs, None, ys_o, np.array([r["objective"] for r in ref]))
+    cert_g = P.certify_batch(lp, costs, None, np.stack([r.y for r in res]), np.array([r.objective for r in res]))
+    assert np.all(np.abs(cert_g[:, 0] - cert_o[:, 0]) <= 1e-5 * (1 + np.abs(cert_o[:, 0])))
+    cert = P.certify_batch(lp, costs, np.stack([r.x for r in res]), np.stack([r.y for r in res]),
+                           np.array([r.objective for r in res]))
+    for k in range(8):
+        assert ref[k]["status"] == 1 and res[k].status == P.OPTIMAL
+        obj = ref[k]["objective"]
+        assert abs(res[k].objective - obj) <= 1e-5 * (1 + abs(obj))
+        relP, relD, gap, _ = host_kkt(lp, res[k].x, res[k].y, costs[k])
+        assert max(relP, relD, gap) <= 2e-6, (k, relP, relD, gap)
+        assert cert[k, 3] == 1.0
+    if prec == P.FP64:
+        for k in range(8):
+            assert abs(res[k].iterations - ref[k]["iterations"]) <= 4 * 64 + 0.05 * ref[k]["iterations"]
+
+
+def test_compaction_keeps_results(gpu):
+    lp = small_refinery()
+    _, costs = P.gen_family(lp, 9, 16, 0.3)
+    on, st_on, _ = P.solve_batch(lp, costs, P.default_options(precision=P.FP64, compact=1))
+    off, st_off, _ = P.solve_batch(lp, costs, P.default_options(precision=P.FP64, compact=0))
+    for a, b in zip(on, off):
+        assert a.status == b.status == P.OPTIMAL
+        assert abs(a.iterations - b.iterations) <= 4 * 64
+        assert abs(a.objective - b.objective) <= 1e-6 * (1 + abs(b.objective))
+    assert st_on["kernel_seconds"] <= st_off["kernel_seconds"] * 1.5
