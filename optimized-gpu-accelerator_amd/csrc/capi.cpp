@@ -139,13 +139,16 @@ int pdhg_solve_batch(const pdhg_lp* base, const pdhg_case* cases, int k, const p
 
 int pdhg_solve_packed(const pdhg_lp* lps, int count, const pdhg_options* opts, pdhg_result* results,
                       pdhg_stats* stats) {
-  (void)lps;
-  (void)count;
-  (void)opts;
-  (void)results;
-  (void)stats;
-  gLastError = "pdhg_solve_packed: packed uneven batches are not available in this build";
-  return -1;
+  return guard([&] {
+    PDHG_REQUIRE(lps && results && count >= 1, "null argument");
+    pdhg_options o;
+    if (opts) o = *opts;
+    else pdhg_default_options(&o);
+    std::vector<pdhgamd::Model> models;
+    models.reserve(count);
+    for (int b = 0; b < count; ++b) models.push_back(pdhgamd::model_from(&lps[b]));
+    pdhgamd::solve_packed(models, o, results, stats);
+  });
 }
 
 int pdhg_prepare(const pdhg_lp* lp, int ruiz_iterations, int pock_chambolle, int rescale, double* R, double* C,

@@ -23,4 +23,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
 
 Model model_from(const pdhg_lp* lp);
 
+// Unrelated LPs advanced together by the persistent packed kernels.
+void solve_packed(const std::vector<Model>& lps, const pdhg_options& o, pdhg_result* results, pdhg_stats* stats);
+
 }  // namespace pdhgamd

@@ -1,0 +1,299 @@
+// Device-side building blocks shared by the batched and the packed PDHG
+// kernels (gfx950): row-to-lane mapping, the row dot products, fixed-order
+// workgroup reductions, per-instance state, and small host helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "device.h"
+
+namespace pdhgamd {
+
+#define HIP_OK(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t _e = (expr);                                                                            \
+    if (_e != hipSuccess)                                                                              \
+      throw ::pdhgamd::Error(std::string("HIP error '") + hipGetErrorString(_e) + "' at " #expr);      \
+  } while (0)
+
+namespace {
+
+
+#ifndef PDHG_LOG_WG
+#define PDHG_LOG_WG 8
+#endif
+constexpr int kLogWG = PDHG_LOG_WG;  // threads per workgroup: 256 (4 wave64)
+constexpr int kWG = 1 << kLogWG;
+constexpr int kWaves = kWG / 64;
+constexpr int kMaxBuckets = 8;
+
+// ---------------------------------------------------------------------------
+// Row mapping
+// ---------------------------------------------------------------------------
+struct RowPlan {
+  int nb;                          // buckets in use
+  int blkStart[kMaxBuckets + 1];   // first workgroup of each bucket
+  int rowStart[kMaxBuckets];       // offset of the bucket in perm (or its first row)
+  int rowCount[kMaxBuckets];
+  int logS[kMaxBuckets];           // log2 lanes per row (per instance)
+  const int* perm;                 // row order, NULL = identity
+  int xcd;                         // 1: contiguous row ranges per XCD
+};
+
+// Workgroup ids are dispatched round-robin over the 8 XCDs; this bijection
+// gives XCD x a contiguous range of logical blocks, so neighbouring rows (which
+// share columns) meet in the same L2.
+__device__ inline int remap_xcd(int bid, int nblk) {
+  const int q = nblk >> 3, r = nblk & 7;
+  const int x = bid & 7, l = bid >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
+}
+
+struct Slot {
+  int row;   // -1: no row for this lane
+  int sub;   // sub-lane within the row group
+  int logS;
+};
+
+template <int LOGB>
+__device__ inline Slot locate(const RowPlan& p, int bid) {
+  int q = 0;
+  while (q + 1 < p.nb && bid >= p.blkStart[q + 1]) ++q;
+  Slot s;
+  s.logS = p.logS[q];
+  const int logG = LOGB + s.logS;
+  const int t = threadIdx.x;
+  const int r = ((bid - p.blkStart[q]) << (kLogWG - logG)) + (t >> logG);
+  s.sub = (t & ((1 << logG) - 1)) >> LOGB;
+  s.row = r < p.rowCount[q] ? (p.perm ? p.perm[p.rowStart[q] + r] : p.rowStart[q] + r) : -1;
+  return s;
+}
+
+__device__ inline int block_id(const RowPlan& p) { return p.xcd ? remap_xcd(blockIdx.x, gridDim.x) : blockIdx.x; }
+
+// Sum over the S sub-lanes of a row group (lanes b, b+B, b+2B, ...).
+template <int LOGB, typename T>
+__device__ inline T reduce_sub(T v, int logS) {
+  for (int o = 1 << LOGB; o < (1 << (LOGB + logS)); o <<= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// sum_e val[e] * x[idx[e]*B + b] over e = e0, e0+step, ... < e1, with four
+// independent chains so the index, value and gather loads of four nonzeros are
+// in flight together.
+template <int LOGB, typename Acc, typename V, typename X>
+__device__ inline Acc row_dot(const int* __restrict__ idx, const V* __restrict__ val, const X* __restrict__ x,
+                              int e0, int e1, int step, int b) {
+  Acc a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int e = e0;
+  for (; e + 3 * step < e1; e += 4 * step) {
+    const int c0 = idx[e], c1 = idx[e + step], c2 = idx[e + 2 * step], c3 = idx[e + 3 * step];
+    const Acc v0 = val[e], v1 = val[e + step], v2 = val[e + 2 * step], v3 = val[e + 3 * step];
+    a0 += v0 * static_cast<Acc>(x[(static_cast<size_t>(c0) << LOGB) + b]);
+    a1 += v1 * static_cast<Acc>(x[(static_cast<size_t>(c1) << LOGB) + b]);
+    a2 += v2 * static_cast<Acc>(x[(static_cast<size_t>(c2) << LOGB) + b]);
+    a3 += v3 * static_cast<Acc>(x[(static_cast<size_t>(c3) << LOGB) + b]);
+  }
+  for (; e < e1; e += step) a0 += static_cast<Acc>(val[e]) * static_cast<Acc>(x[(static_cast<size_t>(idx[e]) << LOGB) + b]);
+  return (a0 + a1) + (a2 + a3);
+}
+
+// B = 64: a whole wavefront serves one row (lane = instance), so the row's
+// bounds are wave-uniform.  Column indices and values then come through the
+// scalar cache (s_load) eight at a time, and the eight 64-lane gathers of
+// a chunk are issued back to back before the first multiply-add waits.  The
+// last partial chunk is predicated: its spare slots re-read the row's first
+// column (a cache hit) with a zero weight, so no serial tail remains.
+template <typename Acc, typename V, typename X>
+__device__ inline Acc row_dot_wave(const int* __restrict__ idx, const V* __restrict__ val, const X* __restrict__ x,
+                                   int e0, int e1, int b) {
+  constexpr int CH = 8;
+  e0 = __builtin_amdgcn_readfirstlane(e0);
+  e1 = __builtin_amdgcn_readfirstlane(e1);
+  Acc a[4] = {0, 0, 0, 0};
+  int e = e0;
+  for (; e + CH <= e1; e += CH) {
+    int c[CH];
+    Acc v[CH], g[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      c[u] = idx[e + u];
+      v[u] = static_cast<Acc>(val[e + u]);
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) g[u] = static_cast<Acc>(x[(static_cast<size_t>(c[u]) << 6) + b]);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) a[u & 3] += v[u] * g[u];
+  }
+  if (e < e1) {
+    const int last = e1 - 1;
+    int c[CH];
+    Acc v[CH], g[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int p = min(e + u, last);  // always a valid entry of this row
+      c[u] = idx[p];
+      const Acc w = static_cast<Acc>(val[p]);
+      v[u] = e + u <= last ? w : static_cast<Acc>(0);
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) g[u] = static_cast<Acc>(x[(static_cast<size_t>(c[u]) << 6) + b]);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) a[u & 3] += v[u] * g[u];
+  }
+  return (a[0] + a[1]) + (a[2] + a[3]);
+}
+
+// Row dot product for the lane layout of LOGB (the B = 64 case uses the
+// wave-uniform path above; there the plan always has one lane per instance).
+template <int LOGB, typename Acc, typename V, typename X>
+__device__ inline Acc row_sum(const int* __restrict__ ptr, const int* __restrict__ idx, const V* __restrict__ val,
+                              const X* __restrict__ x, const Slot& s, int b) {
+  if constexpr (LOGB == 6) {
+    return row_dot_wave<Acc>(idx, val, x, ptr[s.row], ptr[s.row + 1], b);
+  } else {
+    return row_dot<LOGB, Acc>(idx, val, x, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
+  }
+}
+
+// Per-workgroup FP64 partial sums of NQ quantities for each instance
+// b = threadIdx % B, stored at out[((q0 + q) * B + b) * nblk + blk].
+template <int LOGB, int NQ>
+__device__ inline void block_partials(const double (&v)[NQ], double* __restrict__ out, int q0, int nblk, int blk) {
+  constexpr int B = 1 << LOGB;
+  __shared__ double sh[NQ][kWG / 64][B];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double s = v[q];
+    for (int o = 32; o >= B; o >>= 1) s += __shfl_xor(s, o);
+    if (lane < B) sh[q][wave][lane] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < B) {
+    const int b = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double t = sh[q][0][b];
+      for (int w = 1; w < kWaves; ++w) t += sh[q][w][b];
+      out[(static_cast<size_t>(q0 + q) * B + b) * nblk + blk] = t;
+    }
+  }
+}
+
+template <typename T>
+__device__ inline T clamp_to(T v, T lo, T hi) {
+  v = v > lo ? v : lo;  // max(v, lo) then min(., hi): src/pdhg/pdhg.cpp:121-122
+  return v < hi ? v : hi;
+}
+
+// ---------------------------------------------------------------------------
+// Per-instance state (device arrays of length B)
+// ---------------------------------------------------------------------------
+struct Inst {
+  double *tau, *sigma, *omega, *fpR, *fpPrev;
+  long long *iter, *since, *restartAt, *applyAt, *doneIter;
+  int *done, *restarts;
+  const double *cNorm, *rhsNorm;
+  CheckRecord* rec;
+};
+
+// Stride of a per-instance (ld = B, mask = B-1) or shared (ld = 1, mask = 0) array.
+struct Ld {
+  int ld, mask;
+  __device__ size_t at(int i, int b) const { return static_cast<size_t>(i) * ld + (b & mask); }
+};
+
+
+// Check logic of one instance from its nine fixed-order sums S (column side:
+// ||x - x+||^2, ||x+ - x0||^2, dual residual^2, c'x+, column dual objective;
+// row side: primal residual^2, row dual objective, ||y - y+||^2,
+// ||y+ - y0||^2), all FP64 (reference pdhg.cpp:604-656): relative KKT,
+// convergence, the restart rule, the primal-weight update and new steps.
+__device__ inline void finalize_instance(Inst st, int b, const double (&S)[9], const RunScalars& sc, int blockIters,
+                                         int checkEvery) {
+  const double dx2 = S[0], ax2 = S[1], dres2 = S[2], pobjS = S[3], dobjC = S[4];
+  const double pres2 = S[5], dobjR = S[6], dy2 = S[7], ay2 = S[8];
+  const long long it = st.iter[b] + blockIters - 1;
+  const double so = sc.beta * sc.gamma;
+  const double pobj = pobjS * so;
+  const double dobj = (dobjR + dobjC) * so;
+  const double relP = sqrt(pres2) / (1.0 + st.rhsNorm[b]);
+  const double relD = sqrt(dres2) / (1.0 + st.cNorm[b]);
+  const double gap = fabs(pobj - dobj) / (1.0 + fabs(pobj) + fabs(dobj));
+  double omega = st.omega[b];
+  CheckRecord r;
+  r.it = static_cast<double>(it);
+  r.relP = relP;
+  r.relD = relD;
+  r.gap = gap;
+  r.pobj = pobj + sc.offset;
+  r.dobj = dobj + sc.offset;
+  r.omega = omega;
+  r.restarted = 0;
+  st.iter[b] += blockIters;
+  if (relP <= sc.tol && relD <= sc.tol && gap <= sc.tol) {
+    st.done[b] = 1;
+    st.doneIter[b] = it;
+    st.rec[b] = r;
+    return;
+  }
+  long long since = st.since[b] + checkEvery;
+  const double fp = sqrt(omega * dx2 + dy2 / omega);
+  double fpR = st.fpR[b];
+  if (isinf(fpR)) fpR = fp;
+  const bool restart = fp <= 0.2 * fpR || (fp <= 0.8 * fpR && fp > st.fpPrev[b]) ||
+                       since >= static_cast<long long>(0.36 * static_cast<double>(it + 1));
+  double fpPrev = fp;
+  if (restart) {
+    const double dxn = sqrt(ax2), dyn = sqrt(ay2);
+    if (dxn > 1e-10 && dyn > 1e-10) omega = exp(0.5 * log(dyn / dxn) + 0.5 * log(omega));
+    fpR = sqrt(omega * dx2 + dy2 / omega);
+    fpPrev = INFINITY;
+    since = 0;
+    st.restarts[b] += 1;
+    st.omega[b] = omega;
+    st.tau[b] = sc.eta / omega;
+    st.sigma[b] = sc.eta * omega;
+    st.restartAt[b] = it + 1;
+    st.applyAt[b] = it + 1;
+    r.restarted = 1;
+    r.omega = omega;
+  }
+  st.since[b] = since;
+  st.fpR[b] = fpR;
+  st.fpPrev[b] = fpPrev;
+  st.rec[b] = r;
+}
+
+// Host helpers
+int ilog2(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
+// Device buffer list owned by a run; freed together.
+struct Buffers {
+  std::vector<void*> ptrs;
+  ~Buffers() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  T* alloc(size_t count) {
+    void* p = nullptr;
+    HIP_OK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+    ptrs.push_back(p);
+    return static_cast<T*>(p);
+  }
+};
+
+// Host-side row mapping of one CSR matrix for B instances.
+
+}  // namespace
+
+}  // namespace pdhgamd

@@ -1,0 +1,844 @@
+// Packed uneven batches: many unrelated LPs (different A, sizes, scalings)
+// advance together, one restarted-PDHG iteration of all of them per pair of
+// launches.  Their scaled matrices are concatenated (block-diagonal CSR with
+// global column indices), every instance keeps its own restart / primal
+// weight / step state, and the rows are cut into tasks: one workgroup's worth
+// of rows of one instance, grouped by row length.  The kernels are persistent
+// over a device work queue: each workgroup draws task tickets with an atomic
+// until the queue is empty, skips instances that have converged, and exits;
+// the workgroup that draws the last ticket resets the queue for the next
+// launch.  No workgroup ever waits for another, so every wave reaches its exit.
+//
+// The reference solves such LPs one at a time (solvePdhg, src/pdhg/pdhg.h:73);
+// the per-instance algorithm is the same (src/pdhg/pdhg.cpp:483-687).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <thread>
+
+#include "../../include/pdhg_hip.h"
+#include "device.h"
+#include "engine.h"
+#include "kcommon.h"
+#include "prepare.h"
+
+namespace pdhgamd {
+
+namespace {
+
+struct PTask {
+  int inst, start, count, logS;  // rows perm[start .. start+count), 1 << logS lanes per row
+};
+
+struct PScal {  // per-instance scalars
+  const double *eta, *beta, *gamma, *offset;
+};
+
+// Draws the next task ticket for the whole workgroup (-1: queue empty).
+__device__ inline int next_ticket(int* queue, int ntasks) {
+  __shared__ int ticket;
+  __syncthreads();  // previous iteration's readers are done
+  if (threadIdx.x == 0) {
+    const int t = atomicAdd(queue, 1);
+    // Exhausted tickets run from ntasks to ntasks + gridDim.x - 1, one per
+    // workgroup; the holder of the last one resets the queue.
+    if (t == ntasks + static_cast<int>(gridDim.x) - 1) atomicExch(queue, 0);
+    ticket = t;
+  }
+  __syncthreads();
+  return ticket < ntasks ? ticket : -1;
+}
+
+struct PSlot {
+  int row, sub, logS;
+};
+__device__ inline PSlot task_slot(const PTask& t, const int* perm) {
+  PSlot s;
+  s.logS = t.logS;
+  const int r = threadIdx.x >> t.logS;
+  s.sub = threadIdx.x & ((1 << t.logS) - 1);
+  s.row = r < t.count ? perm[t.start + r] : -1;
+  return s;
+}
+
+template <typename T, bool CHECK>
+__global__ __launch_bounds__(kWG) void k_pcol(const PTask* __restrict__ tasks, int ntasks, const int* __restrict__ perm,
+                                              int* queue, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                              const T* __restrict__ val, const T* __restrict__ y,
+                                              const T* __restrict__ yp, T* __restrict__ x, T* __restrict__ xp,
+                                              T* __restrict__ x0, const T* __restrict__ c, const T* __restrict__ lo,
+                                              const T* __restrict__ up, Inst st, int tloc, double rho,
+                                              double* __restrict__ part) {
+  for (int tk; (tk = next_ticket(queue, ntasks)) >= 0;) {
+    const PTask task = tasks[tk];
+    const int b = task.inst;
+    if (st.done[b]) continue;
+    const PSlot s = task_slot(task, perm);
+    const long long it = st.iter[b] + tloc;
+    const bool rst = st.applyAt[b] == it;
+    const T* __restrict__ ysrc = rst ? yp : y;
+    T acc = 0;
+    if (s.row >= 0) acc = row_dot<0, T>(idx, val, ysrc, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, 0);
+    acc = reduce_sub<0>(acc, s.logS);
+    double q[2] = {0.0, 0.0};
+    if (s.row >= 0 && s.sub == 0) {
+      const int j = s.row;
+      const T xin = rst ? xp[j] : x[j];
+      const T xo = rst ? xin : x0[j];
+      if (rst) x0[j] = xin;
+      const T v = clamp_to<T>(xin - static_cast<T>(st.tau[b]) * (c[j] - acc), lo[j], up[j]);
+      const double kk = static_cast<double>(it - st.restartAt[b]);
+      const double lam = (kk + 1.0) / (kk + 2.0);
+      x[j] = static_cast<T>(lam) * (static_cast<T>(1.0 + rho) * v - static_cast<T>(rho) * xin) +
+             static_cast<T>(1.0 - lam) * xo;
+      xp[j] = v;
+      if (CHECK) {
+        const double d = static_cast<double>(xin) - static_cast<double>(v);
+        const double e = static_cast<double>(v) - static_cast<double>(xo);
+        q[0] = d * d;
+        q[1] = e * e;
+      }
+    }
+    if (CHECK) block_partials<0, 2>(q, part, 0, ntasks, tk);
+  }
+}
+
+template <typename T, bool CHECK>
+__global__ __launch_bounds__(kWG) void k_prow(const PTask* __restrict__ tasks, int ntasks, const int* __restrict__ perm,
+                                              int* queue, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                              const T* __restrict__ val, const double* __restrict__ val64,
+                                              const T* __restrict__ xp, T* __restrict__ y, T* __restrict__ yp,
+                                              T* __restrict__ y0, T* __restrict__ ax, T* __restrict__ axp,
+                                              T* __restrict__ ax0, const T* __restrict__ rl, const T* __restrict__ ru,
+                                              const double* __restrict__ rl64, const double* __restrict__ ru64,
+                                              const double* __restrict__ rfac, Inst st, int tloc, double rho,
+                                              double* __restrict__ part) {
+  for (int tk; (tk = next_ticket(queue, ntasks)) >= 0;) {
+    const PTask task = tasks[tk];
+    const int b = task.inst;
+    if (st.done[b]) continue;
+    const PSlot s = task_slot(task, perm);
+    T acc = 0;
+    double acc64 = 0;
+    if (s.row >= 0) {
+      const int e0 = ptr[s.row] + s.sub, e1 = ptr[s.row + 1], step = 1 << s.logS;
+      acc = row_dot<0, T>(idx, val, xp, e0, e1, step, 0);
+      if (CHECK) acc64 = row_dot<0, double>(idx, val64, xp, e0, e1, step, 0);
+    }
+    acc = reduce_sub<0>(acc, s.logS);
+    if (CHECK) acc64 = reduce_sub<0>(acc64, s.logS);
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+    if (s.row >= 0 && s.sub == 0) {
+      const int i = s.row;
+      const long long it = st.iter[b] + tloc;
+      const bool rst = st.applyAt[b] == it;
+      const T yin = rst ? yp[i] : y[i];
+      const T axin = rst ? axp[i] : ax[i];
+      const T yo = rst ? yin : y0[i];
+      const T axo = rst ? axin : ax0[i];
+      if (rst) {
+        y0[i] = yin;
+        ax0[i] = axin;
+      }
+      // Reflected dual step in the cancellation-free form y+ = y - sigma (g - w).
+      const T sg = static_cast<T>(st.sigma[b]);
+      const T g = static_cast<T>(2) * acc - axin;
+      const T tq = yin / sg;
+      const T dlo = g - rl[i], dhi = g - ru[i];
+      const T yplus = dlo < tq ? yin - sg * dlo : (dhi > tq ? yin - sg * dhi : static_cast<T>(0));
+      const double kk = static_cast<double>(it - st.restartAt[b]);
+      const double lam = (kk + 1.0) / (kk + 2.0);
+      const T lamT = static_cast<T>(lam), omT = static_cast<T>(1.0 - lam);
+      const T r1 = static_cast<T>(1.0 + rho), r0 = static_cast<T>(rho);
+      y[i] = lamT * (r1 * yplus - r0 * yin) + omT * yo;
+      ax[i] = lamT * (r1 * acc - r0 * axin) + omT * axo;
+      if (CHECK) {
+        yp[i] = yplus;
+        axp[i] = acc;
+        const double L = rl64[i], U = ru64[i];
+        const double res = (fmax(L - acc64, 0.0) + fmax(acc64 - U, 0.0)) * rfac[i];
+        const double yy = static_cast<double>(yplus);
+        q[0] = res * res;
+        if (yy > 0 && L > -1e300) q[1] = yy * L;
+        else if (yy < 0 && U < 1e300) q[1] = yy * U;
+        const double d = static_cast<double>(yin) - yy, e = yy - static_cast<double>(yo);
+        q[2] = d * d;
+        q[3] = e * e;
+      }
+    }
+    if (CHECK) block_partials<0, 4>(q, part, 0, ntasks, tk);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_pcolcheck(const PTask* __restrict__ tasks, int ntasks,
+                                                   const int* __restrict__ perm, int* queue,
+                                                   const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                   const double* __restrict__ val64, const T* __restrict__ yp,
+                                                   const T* __restrict__ xp, const double* __restrict__ c64,
+                                                   const double* __restrict__ lo64, const double* __restrict__ up64,
+                                                   const double* __restrict__ cfac, Inst st,
+                                                   double* __restrict__ part) {
+  for (int tk; (tk = next_ticket(queue, ntasks)) >= 0;) {
+    const PTask task = tasks[tk];
+    if (st.done[task.inst]) continue;
+    const PSlot s = task_slot(task, perm);
+    double acc = 0;
+    if (s.row >= 0) acc = row_dot<0, double>(idx, val64, yp, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, 0);
+    acc = reduce_sub<0>(acc, s.logS);
+    double q[3] = {0.0, 0.0, 0.0};
+    if (s.row >= 0 && s.sub == 0) {
+      const int j = s.row;
+      const double cj = c64[j], l = lo64[j], u = up64[j];
+      const double d = cj - acc;
+      const bool hl = l > -1e300, hu = u < 1e300;
+      double res = 0;
+      if (!hl && !hu) res = fabs(d);
+      else if (!hl) res = fmax(d, 0.0);
+      else if (!hu) res = fmax(-d, 0.0);
+      res *= cfac[j];
+      q[0] = res * res;
+      q[1] = cj * static_cast<double>(xp[j]);
+      if (d > 0 && hl) q[2] = d * l;
+      else if (d < 0 && hu) q[2] = d * u;
+    }
+    block_partials<0, 3>(q, part, 2, ntasks, tk);
+  }
+}
+
+// One workgroup per instance: fixed-order sums over the instance's own task
+// ranges, then the shared FP64 check logic with that instance's scalars.
+__global__ __launch_bounds__(kWG) void k_pfinalize(Inst st, const double* __restrict__ colPart, int nColTasks,
+                                                   const int* __restrict__ colBeg, const double* __restrict__ rowPart,
+                                                   int nRowTasks, const int* __restrict__ rowBeg, PScal ps,
+                                                   double tol, int blockIters, int checkEvery) {
+  const int b = blockIdx.x;
+  if (st.done[b]) return;
+  double v[9];
+  for (int q = 0; q < 5; ++q) {
+    const double* p = colPart + static_cast<size_t>(q) * nColTasks;
+    double s = 0;
+    for (int k = colBeg[b] + threadIdx.x; k < colBeg[b + 1]; k += kWG) s += p[k];
+    v[q] = s;
+  }
+  for (int q = 0; q < 4; ++q) {
+    const double* p = rowPart + static_cast<size_t>(q) * nRowTasks;
+    double s = 0;
+    for (int k = rowBeg[b] + threadIdx.x; k < rowBeg[b + 1]; k += kWG) s += p[k];
+    v[5 + q] = s;
+  }
+  __shared__ double sh[9][kWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int q = 0; q < 9; ++q) {
+    double s = v[q];
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) sh[q][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double S[9];
+  for (int q = 0; q < 9; ++q) {
+    S[q] = sh[q][0];
+    for (int w = 1; w < kWaves; ++w) S[q] += sh[q][w];
+  }
+  RunScalars sc;
+  sc.eta = ps.eta[b];
+  sc.beta = ps.beta[b];
+  sc.gamma = ps.gamma[b];
+  sc.offset = ps.offset[b];
+  sc.tol = tol;
+  finalize_instance(st, b, S, sc, blockIters, checkEvery);
+}
+
+// out = A in over all tasks (initial activities).
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_pspmv(const PTask* __restrict__ tasks, int ntasks, const int* __restrict__ perm,
+                                               int* queue, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                               const T* __restrict__ val, const T* __restrict__ in, T* __restrict__ out) {
+  for (int tk; (tk = next_ticket(queue, ntasks)) >= 0;) {
+    const PSlot s = task_slot(tasks[tk], perm);
+    T acc = 0;
+    if (s.row >= 0) acc = row_dot<0, T>(idx, val, in, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, 0);
+    acc = reduce_sub<0>(acc, s.logS);
+    if (s.row >= 0 && s.sub == 0) out[s.row] = acc;
+  }
+}
+
+// Tasks of a block-diagonal CSR: per instance, rows bucketed by the lane count
+// nextpow2(ceil(len / nnzPerLane)) (capped at 64), each task one workgroup's
+// worth of rows of one bucket of one instance.  beg[b] .. beg[b+1] are the
+// tasks of instance b.
+void build_tasks(const std::vector<int>& ptr, const std::vector<int>& rowOff, int nnzPerLane, std::vector<PTask>& tasks,
+                 std::vector<int>& perm, std::vector<int>& beg) {
+  const int K = static_cast<int>(rowOff.size()) - 1;
+  tasks.clear();
+  perm.clear();
+  beg.assign(K + 1, 0);
+  for (int b = 0; b < K; ++b) {
+    beg[b] = static_cast<int>(tasks.size());
+    std::vector<std::vector<int>> bucket(7);
+    for (int r = rowOff[b]; r < rowOff[b + 1]; ++r) {
+      const int len = ptr[r + 1] - ptr[r];
+      const int want = (len + nnzPerLane - 1) / std::max(1, nnzPerLane);
+      bucket[std::min(6, ilog2(std::max(1, want)))].push_back(r);
+    }
+    for (int l = 0; l <= 6; ++l) {
+      const int per = kWG >> l;
+      for (size_t a = 0; a < bucket[l].size(); a += per) {
+        PTask t;
+        t.inst = b;
+        t.start = static_cast<int>(perm.size());
+        t.count = static_cast<int>(std::min<size_t>(per, bucket[l].size() - a));
+        t.logS = l;
+        perm.insert(perm.end(), bucket[l].begin() + a, bucket[l].begin() + a + t.count);
+        tasks.push_back(t);
+      }
+    }
+  }
+  beg[K] = static_cast<int>(tasks.size());
+}
+
+template <typename T>
+class PackedRun {
+ public:
+  // A, AT: concatenated CSRs (global indices); rowOff / colOff: instance offsets.
+  PackedRun(int device, const Csr& A, const Csr& AT, const std::vector<int>& rowOff, const std::vector<int>& colOff,
+            const std::vector<double>& rfac, const std::vector<double>& cfac, int nnzPerLane, int checkEvery,
+            double rho, bool useGraph)
+      : M_(A.rows), N_(AT.rows), K_(static_cast<int>(rowOff.size()) - 1), checkEvery_(checkEvery), rho_(rho),
+        useGraph_(useGraph) {
+    HIP_OK(hipSetDevice(device));
+    HIP_OK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    HIP_OK(hipEventCreate(&e0_));
+    HIP_OK(hipEventCreate(&e1_));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, device));
+    grid_ = std::max(1, prop.multiProcessorCount) * 4;
+    std::vector<PTask> ct, rt;
+    std::vector<int> cperm, rperm;
+    build_tasks(AT.ptr, colOff, nnzPerLane, ct, cperm, colBeg_);
+    build_tasks(A.ptr, rowOff, nnzPerLane, rt, rperm, rowBeg_);
+    nCol_ = static_cast<int>(ct.size());
+    nRow_ = static_cast<int>(rt.size());
+    colTasks_ = up(ct);
+    rowTasks_ = up(rt);
+    colPerm_ = up(cperm);
+    rowPerm_ = up(rperm);
+    dColBeg_ = up(colBeg_);
+    dRowBeg_ = up(rowBeg_);
+    aPtr_ = up(A.ptr);
+    aIdx_ = up(A.idx);
+    tPtr_ = up(AT.ptr);
+    tIdx_ = up(AT.idx);
+    aVal64_ = up(A.val);
+    tVal64_ = up(AT.val);
+    if (sizeof(T) == 8) {
+      aVal_ = reinterpret_cast<T*>(aVal64_);
+      tVal_ = reinterpret_cast<T*>(tVal64_);
+    } else {
+      aVal_ = upT(A.val);
+      tVal_ = upT(AT.val);
+    }
+    rfac_ = up(rfac);
+    cfac_ = up(cfac);
+    for (T** v : {&x_, &xp_, &x0_, &c_, &lo_, &up_}) *v = buf_.alloc<T>(N_);
+    for (T** v : {&y_, &yp_, &y0_, &ax_, &axp_, &ax0_, &rl_, &ru_}) *v = buf_.alloc<T>(M_);
+    for (double** v : {&c64_, &lo64_, &up64_}) *v = buf_.alloc<double>(N_);
+    for (double** v : {&rl64_, &ru64_}) *v = buf_.alloc<double>(M_);
+    colPart_ = buf_.alloc<double>(static_cast<size_t>(5) * nCol_);
+    rowPart_ = buf_.alloc<double>(static_cast<size_t>(4) * nRow_);
+    queue_ = buf_.alloc<int>(4);
+    HIP_OK(hipMemset(queue_, 0, 4 * sizeof(int)));
+    st_.tau = buf_.alloc<double>(K_);
+    st_.sigma = buf_.alloc<double>(K_);
+    st_.omega = buf_.alloc<double>(K_);
+    st_.fpR = buf_.alloc<double>(K_);
+    st_.fpPrev = buf_.alloc<double>(K_);
+    st_.iter = buf_.alloc<long long>(K_);
+    st_.since = buf_.alloc<long long>(K_);
+    st_.restartAt = buf_.alloc<long long>(K_);
+    st_.applyAt = buf_.alloc<long long>(K_);
+    st_.doneIter = buf_.alloc<long long>(K_);
+    st_.done = buf_.alloc<int>(K_);
+    st_.restarts = buf_.alloc<int>(K_);
+    cNorm_ = buf_.alloc<double>(K_);
+    rhsNorm_ = buf_.alloc<double>(K_);
+    st_.cNorm = cNorm_;
+    st_.rhsNorm = rhsNorm_;
+    st_.rec = buf_.alloc<CheckRecord>(K_);
+    for (double** v : {&eta_, &beta_, &gamma_, &offset_}) *v = buf_.alloc<double>(K_);
+    ps_ = PScal{eta_, beta_, gamma_, offset_};
+  }
+  ~PackedRun() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    (void)hipEventDestroy(e0_);
+    (void)hipEventDestroy(e1_);
+    (void)hipStreamDestroy(s_);
+  }
+
+  void upload(const std::vector<double>& c, const std::vector<double>& lo, const std::vector<double>& upv,
+              const std::vector<double>& rl, const std::vector<double>& ru, const std::vector<double>& x,
+              const std::vector<double>& y) {
+    putT(c_, c);
+    putT(lo_, lo);
+    putT(up_, upv);
+    putT(rl_, rl);
+    putT(ru_, ru);
+    put64(c64_, c);
+    put64(lo64_, lo);
+    put64(up64_, upv);
+    put64(rl64_, rl);
+    put64(ru64_, ru);
+    for (T* v : {x_, x0_, xp_}) putT(v, x);
+    for (T* v : {y_, y0_, yp_}) putT(v, y);
+  }
+
+  void init_state(const std::vector<double>& omega, const std::vector<double>& cNorm,
+                  const std::vector<double>& rhsNorm, const std::vector<double>& eta, const std::vector<double>& beta,
+                  const std::vector<double>& gamma, const std::vector<double>& offset, double tol) {
+    tol_ = tol;
+    std::vector<double> tau(K_), sigma(K_), inf(K_, INFINITY);
+    for (int b = 0; b < K_; ++b) {
+      tau[b] = eta[b] / omega[b];
+      sigma[b] = eta[b] * omega[b];
+    }
+    std::vector<long long> zero(K_, 0), neg(K_, -1);
+    std::vector<int> zi(K_, 0);
+    put64(st_.tau, tau);
+    put64(st_.sigma, sigma);
+    put64(st_.omega, omega);
+    put64(st_.fpR, inf);
+    put64(st_.fpPrev, inf);
+    putv(st_.iter, zero);
+    putv(st_.since, zero);
+    putv(st_.restartAt, zero);
+    putv(st_.applyAt, neg);
+    putv(st_.doneIter, neg);
+    putv(st_.done, zi);
+    putv(st_.restarts, zi);
+    put64(cNorm_, cNorm);
+    put64(rhsNorm_, rhsNorm);
+    put64(eta_, eta);
+    put64(beta_, beta);
+    put64(gamma_, gamma);
+    put64(offset_, offset);
+  }
+
+  void start() {
+    hipLaunchKernelGGL(k_pspmv<T>, dim3(grid_), dim3(kWG), 0, s_, rowTasks_, nRow_, rowPerm_, queue_ + 3, aPtr_, aIdx_,
+                       aVal_, x_, ax_);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(ax0_, ax_, sizeof(T) * M_, hipMemcpyDeviceToDevice, s_));
+    HIP_OK(hipMemcpyAsync(axp_, ax_, sizeof(T) * M_, hipMemcpyDeviceToDevice, s_));
+    HIP_OK(hipEventRecord(e0_, s_));
+  }
+
+  void run_block(int iters) {
+    if (useGraph_ && iters == checkEvery_) {
+      if (!exec_) {
+        HIP_OK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        enqueue(iters);
+        HIP_OK(hipStreamEndCapture(s_, &graph_));
+        HIP_OK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+      }
+      HIP_OK(hipGraphLaunch(exec_, s_));
+      graphLaunches++;
+    } else {
+      enqueue(iters);
+    }
+    launches += 2 * iters + 2;
+  }
+
+  void fetch(std::vector<CheckRecord>& rec, std::vector<int>& done, std::vector<long long>& doneIter,
+             std::vector<int>& restarts) {
+    rec.resize(K_);
+    done.resize(K_);
+    doneIter.resize(K_);
+    restarts.resize(K_);
+    HIP_OK(hipMemcpyAsync(rec.data(), st_.rec, K_ * sizeof(CheckRecord), hipMemcpyDeviceToHost, s_));
+    HIP_OK(hipMemcpyAsync(done.data(), st_.done, K_ * sizeof(int), hipMemcpyDeviceToHost, s_));
+    HIP_OK(hipMemcpyAsync(doneIter.data(), st_.doneIter, K_ * sizeof(long long), hipMemcpyDeviceToHost, s_));
+    HIP_OK(hipMemcpyAsync(restarts.data(), st_.restarts, K_ * sizeof(int), hipMemcpyDeviceToHost, s_));
+    HIP_OK(hipStreamSynchronize(s_));
+  }
+
+  void download(bool primal, std::vector<double>& out) {
+    const int len = primal ? N_ : M_;
+    std::vector<T> t(len);
+    HIP_OK(hipMemcpyAsync(t.data(), primal ? xp_ : yp_, len * sizeof(T), hipMemcpyDeviceToHost, s_));
+    HIP_OK(hipStreamSynchronize(s_));
+    out.assign(t.begin(), t.end());
+  }
+
+  double stop() {
+    HIP_OK(hipEventRecord(e1_, s_));
+    HIP_OK(hipEventSynchronize(e1_));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, e0_, e1_));
+    return ms / 1000.0;
+  }
+
+  long long launches = 0, graphLaunches = 0;
+  int tasks() const { return nCol_ + nRow_; }
+
+ private:
+  void enqueue(int iters) {
+    for (int t = 0; t < iters; ++t) {
+      const bool chk = t == iters - 1;
+      auto colK = chk ? k_pcol<T, true> : k_pcol<T, false>;
+      auto rowK = chk ? k_prow<T, true> : k_prow<T, false>;
+      hipLaunchKernelGGL(colK, dim3(grid_), dim3(kWG), 0, s_, colTasks_, nCol_,
+                         colPerm_, queue_ + 0, tPtr_, tIdx_, tVal_, y_, yp_, x_, xp_, x0_, c_, lo_, up_, st_, t, rho_,
+                         colPart_);
+      hipLaunchKernelGGL(rowK, dim3(grid_), dim3(kWG), 0, s_, rowTasks_, nRow_,
+                         rowPerm_, queue_ + 1, aPtr_, aIdx_, aVal_, aVal64_, xp_, y_, yp_, y0_, ax_, axp_, ax0_, rl_,
+                         ru_, rl64_, ru64_, rfac_, st_, t, rho_, rowPart_);
+    }
+    hipLaunchKernelGGL(k_pcolcheck<T>, dim3(grid_), dim3(kWG), 0, s_, colTasks_, nCol_, colPerm_, queue_ + 2, tPtr_,
+                       tIdx_, tVal64_, yp_, xp_, c64_, lo64_, up64_, cfac_, st_, colPart_);
+    hipLaunchKernelGGL(k_pfinalize, dim3(K_), dim3(kWG), 0, s_, st_, colPart_, nCol_, dColBeg_, rowPart_, nRow_,
+                       dRowBeg_, ps_, tol_, iters, checkEvery_);
+    HIP_OK(hipGetLastError());
+  }
+  template <typename U>
+  U* up(const std::vector<U>& v) {
+    U* d = buf_.alloc<U>(v.size());
+    HIP_OK(hipMemcpy(d, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice));
+    return d;
+  }
+  T* upT(const std::vector<double>& v) {
+    std::vector<T> t(v.begin(), v.end());
+    return up(t);
+  }
+  void putT(T* d, const std::vector<double>& v) {
+    std::vector<T> t(v.begin(), v.end());
+    HIP_OK(hipMemcpy(d, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void put64(double* d, const std::vector<double>& v) {
+    HIP_OK(hipMemcpy(d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  template <typename U>
+  void putv(U* d, const std::vector<U>& v) {
+    HIP_OK(hipMemcpy(d, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice));
+  }
+
+  int M_, N_, K_, checkEvery_;
+  double rho_;
+  bool useGraph_;
+  int grid_ = 1, nCol_ = 0, nRow_ = 0;
+  double tol_ = 1e-6;
+  hipStream_t s_ = nullptr;
+  hipEvent_t e0_ = nullptr, e1_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  Buffers buf_;
+  std::vector<int> colBeg_, rowBeg_;
+  PTask *colTasks_ = nullptr, *rowTasks_ = nullptr;
+  int *colPerm_ = nullptr, *rowPerm_ = nullptr, *dColBeg_ = nullptr, *dRowBeg_ = nullptr, *queue_ = nullptr;
+  int *aPtr_ = nullptr, *aIdx_ = nullptr, *tPtr_ = nullptr, *tIdx_ = nullptr;
+  T *aVal_ = nullptr, *tVal_ = nullptr;
+  double *aVal64_ = nullptr, *tVal64_ = nullptr, *rfac_ = nullptr, *cfac_ = nullptr;
+  T *x_, *xp_, *x0_, *c_, *lo_, *up_, *y_, *yp_, *y0_, *ax_, *axp_, *ax0_, *rl_, *ru_;
+  double *c64_, *lo64_, *up64_, *rl64_, *ru64_, *colPart_, *rowPart_, *cNorm_, *rhsNorm_;
+  double *eta_, *beta_, *gamma_, *offset_;
+  Inst st_{};
+  PScal ps_{};
+};
+
+using Clock = std::chrono::steady_clock;
+
+// Scaled data of a subset of the instances, concatenated block-diagonally
+// (global row / column indices); slot q of the run is instance ids[q].
+struct PackedData {
+  std::vector<int> ids, rowOff, colOff;
+  Csr A, AT;
+  std::vector<double> rfac, cfac, c, lo, upv, rl, ru, x, y;
+  std::vector<double> omega, cNorm, rhsNorm, eta, beta, gamma, offset;
+};
+
+// What one packed run leaves behind, per slot (iterates in the scaled space).
+struct PackedOutcome {
+  std::vector<int> done, restarts;
+  std::vector<long long> doneIter;
+  std::vector<CheckRecord> rec;
+  std::vector<double> xs, ys;
+  long long it = 0, launches = 0, graphLaunches = 0;
+  bool expired = false, stalled = false;
+  double kernel = 0;
+};
+
+// xw / yw / ww (indexed by instance): a starting point in the scaled space and
+// the primal weight to keep; null gives the reference's start, x0 = clamp(0),
+// y0 = 0 and the weight ||c|| / ||b|| (src/pdhg/pdhg.cpp:509-545).
+PackedData pack(const std::vector<Model>& lps, const std::vector<Prepared>& P, const std::vector<int>& ids,
+                const std::vector<std::vector<double>>* xw, const std::vector<std::vector<double>>* yw,
+                const std::vector<double>* ww) {
+  PackedData d;
+  const int K = static_cast<int>(ids.size());
+  d.ids = ids;
+  d.rowOff.assign(K + 1, 0);
+  d.colOff.assign(K + 1, 0);
+  for (int q = 0; q < K; ++q) {
+    d.rowOff[q + 1] = d.rowOff[q] + P[ids[q]].m;
+    d.colOff[q + 1] = d.colOff[q] + P[ids[q]].n;
+  }
+  const int M = d.rowOff[K], N = d.colOff[K];
+  Csr& A = d.A;
+  Csr& AT = d.AT;
+  A.rows = M;
+  A.cols = N;
+  AT.rows = N;
+  AT.cols = M;
+  A.ptr.assign(1, 0);
+  AT.ptr.assign(1, 0);
+  std::vector<double>&rfac = d.rfac, &cfac = d.cfac, &c = d.c, &lo = d.lo, &upv = d.upv, &rl = d.rl, &ru = d.ru,
+  &x = d.x, &y = d.y;
+  y.reserve(M);
+  std::vector<double>&omega = d.omega, &cNorm = d.cNorm, &rhsNorm = d.rhsNorm, &eta = d.eta, &beta = d.beta,
+  &gamma = d.gamma, &offset = d.offset;
+  for (std::vector<double>* v : {&cNorm, &rhsNorm, &eta, &beta, &gamma, &offset}) v->assign(K, 0.0);
+  omega.assign(K, 1.0);
+  const std::vector<int>& rowOff = d.rowOff;
+  const std::vector<int>& colOff = d.colOff;
+  for (int q = 0; q < K; ++q) {
+    const int b = ids[q];
+    const Prepared& p = P[b];
+    const Model& lp = lps[b];
+    for (int i = 0; i < p.m; ++i) {
+      for (int e = p.A.ptr[i]; e < p.A.ptr[i + 1]; ++e) {
+        A.idx.push_back(colOff[q] + p.A.idx[e]);
+        A.val.push_back(p.A.val[e]);
+      }
+      A.ptr.push_back(static_cast<int>(A.idx.size()));
+    }
+    for (int j = 0; j < p.n; ++j) {
+      for (int e = p.AT.ptr[j]; e < p.AT.ptr[j + 1]; ++e) {
+        AT.idx.push_back(rowOff[q] + p.AT.idx[e]);
+        AT.val.push_back(p.AT.val[e]);
+      }
+      AT.ptr.push_back(static_cast<int>(AT.idx.size()));
+    }
+    rfac.insert(rfac.end(), p.rfac.begin(), p.rfac.end());
+    cfac.insert(cfac.end(), p.cfac.begin(), p.cfac.end());
+    // Scaled case data and initial point (src/pdhg/pdhg.cpp:511-545).
+    double cn0 = 0, bn0 = 0, cn = 0, bn = 0;
+    for (int j = 0; j < p.n; ++j) {
+      const double cj = lp.c[j] * p.C[j] / p.gamma;
+      const double lj = finite_bound(lp.l[j]) ? lp.l[j] / p.C[j] / p.beta : -HUGE_VAL;
+      const double uj = finite_bound(lp.u[j]) ? lp.u[j] / p.C[j] / p.beta : HUGE_VAL;
+      c.push_back(cj);
+      lo.push_back(lj);
+      upv.push_back(uj);
+      x.push_back(std::min(std::max(xw ? (*xw)[b][j] : 0.0, lj), uj));
+      cn0 += lp.c[j] * lp.c[j];
+      cn += cj * cj;
+    }
+    for (int i = 0; i < p.m; ++i) {
+      const double li = finite_bound(lp.L[i]) ? lp.L[i] * p.R[i] / p.beta : -HUGE_VAL;
+      const double ui = finite_bound(lp.U[i]) ? lp.U[i] * p.R[i] / p.beta : HUGE_VAL;
+      rl.push_back(li);
+      ru.push_back(ui);
+      y.push_back(yw ? (*yw)[b][i] : 0.0);
+      if (finite_bound(lp.L[i])) bn0 += lp.L[i] * lp.L[i];
+      if (finite_bound(lp.U[i]) && lp.U[i] != lp.L[i]) bn0 += lp.U[i] * lp.U[i];
+      if (finite_bound(li)) bn += li * li;
+      if (finite_bound(ui) && ui != li) bn += ui * ui;
+    }
+    cNorm[q] = std::sqrt(cn0);
+    rhsNorm[q] = std::sqrt(bn0);
+    if (ww) omega[q] = (*ww)[b];
+    else if (cn > 1e-20 && bn > 1e-20) omega[q] = std::sqrt(cn) / std::sqrt(bn);
+    eta[q] = 0.998 / p.normA;
+    beta[q] = p.beta;
+    gamma[q] = p.gamma;
+    offset[q] = p.offset;
+  }
+  PDHG_REQUIRE(A.idx.size() < static_cast<size_t>(INT32_MAX), "packed batch exceeds 2^31 nonzeros");
+  return d;
+}
+
+// One packed run of at most maxIter iterations.  With stallRule (FP32
+// iterates), an instance whose best relative KKT has not halved over the last
+// 256 checks is stalled at the FP32 resolution floor; once half of the live
+// instances are, the run ends with stalled = true.
+template <typename T>
+PackedOutcome run_packed(const PackedData& d, const pdhg_options& o, long long maxIter, Clock::time_point t0,
+                         bool stallRule) {
+  const int K = static_cast<int>(d.ids.size());
+  PackedRun<T> run(o.device, d.A, d.AT, d.rowOff, d.colOff, d.rfac, d.cfac, o.lanes_per_row > 0 ? o.lanes_per_row : 4,
+                   o.check_every, o.reflection, o.use_graph != 0);
+  run.upload(d.c, d.lo, d.upv, d.rl, d.ru, d.x, d.y);
+  run.init_state(d.omega, d.cNorm, d.rhsNorm, d.eta, d.beta, d.gamma, d.offset, o.tolerance);
+  run.start();
+  PackedOutcome r;
+  std::vector<double> refKkt(K, INFINITY);
+  std::vector<long long> lastGain(K, 0);
+  long long checks = 0;
+  auto lastProgress = Clock::now();
+  while (r.it < maxIter) {
+    const int nIt = static_cast<int>(std::min<long long>(o.check_every, maxIter - r.it));
+    run.run_block(nIt);
+    run.fetch(r.rec, r.done, r.doneIter, r.restarts);
+    r.it += nIt;
+    ++checks;
+    int remaining = 0, stalled = 0;
+    for (int q = 0; q < K; ++q) {
+      if (r.done[q]) continue;
+      ++remaining;
+      const CheckRecord& c = r.rec[q];
+      const double kkt = std::max(c.relP, std::max(c.relD, c.gap));
+      if (kkt < 0.5 * refKkt[q]) {
+        refKkt[q] = kkt;
+        lastGain[q] = checks;
+      }
+      if (checks - lastGain[q] >= 256) ++stalled;
+    }
+    if (o.progress_seconds > 0 &&
+        std::chrono::duration<double>(Clock::now() - lastProgress).count() >= o.progress_seconds) {
+      lastProgress = Clock::now();
+      std::fprintf(stderr, "  pdhg packed: iteration %lld, %d of %d live\n", r.it, remaining, K);
+    }
+    if (remaining == 0) break;
+    if (o.time_limit > 0 && std::chrono::duration<double>(Clock::now() - t0).count() >= o.time_limit) {
+      r.expired = true;
+      break;
+    }
+    if (stallRule && stalled > 0 && 2 * stalled >= remaining) {
+      r.stalled = true;
+      break;
+    }
+  }
+  r.kernel = run.stop();
+  run.download(true, r.xs);
+  run.download(false, r.ys);
+  r.launches = run.launches;
+  r.graphLaunches = run.graphLaunches;
+  return r;
+}
+
+// Results of the slots of one run, in the original space; iterations and
+// restarts continue from itBase / restartBase (indexed by instance).
+void write_results(const PackedData& d, const PackedOutcome& r, const std::vector<Prepared>& P, long long itBase,
+                   const std::vector<int>& restartBase, pdhg_result* results) {
+  for (size_t q = 0; q < d.ids.size(); ++q) {
+    const int b = d.ids[q];
+    pdhg_result& out = results[b];
+    const CheckRecord& c = r.rec[q];
+    out.status = r.done[q] ? PDHG_OPTIMAL : (r.expired ? PDHG_TIME_LIMIT : PDHG_ITERATION_LIMIT);
+    out.iterations = itBase + (r.done[q] ? r.doneIter[q] : (r.expired ? r.it - 1 : r.it));
+    out.restarts = restartBase[b] + r.restarts[q];
+    out.objective = c.pobj;
+    out.dual_objective = c.dobj;
+    out.rel_primal = c.relP;
+    out.rel_dual = c.relD;
+    out.rel_gap = c.gap;
+    out.relative_kkt = std::max(c.relP, std::max(c.relD, c.gap));
+    out.primal_weight = c.omega;
+    const Prepared& p = P[b];
+    if (out.x)
+      for (int j = 0; j < p.n; ++j) out.x[j] = r.xs[d.colOff[q] + j] * p.C[j] * p.beta;
+    if (out.row_dual)
+      for (int i = 0; i < p.m; ++i) out.row_dual[i] = r.ys[d.rowOff[q] + i] * p.R[i] * p.gamma;
+  }
+}
+
+void solve_packed_all(const std::vector<Model>& lps, const pdhg_options& o, pdhg_result* results,
+                      pdhg_stats* stats) {
+  const auto t0 = Clock::now();
+  const int K = static_cast<int>(lps.size());
+  // Set-up of every instance (src/pdhg/pdhg.cpp:424-459), on host threads.
+  std::vector<Prepared> P(K);
+  {
+    const int nt = std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    std::vector<std::string> err(nt);
+    for (int t = 0; t < nt; ++t)
+      pool.emplace_back([&, t] {
+        try {
+          for (int b = t; b < K; b += nt) P[b] = prepare(lps[b], o.ruiz_iterations, o.pock_chambolle != 0, o.rescale != 0);
+        } catch (const std::exception& e) {
+          err[t] = e.what();
+        }
+      });
+    for (auto& th : pool) th.join();
+    for (auto& e : err) PDHG_REQUIRE(e.empty(), e);
+  }
+  std::vector<int> all(K);
+  for (int b = 0; b < K; ++b) all[b] = b;
+  const PackedData d = pack(lps, P, all, nullptr, nullptr, nullptr);
+  const double setup = std::chrono::duration<double>(Clock::now() - t0).count();
+  const auto tSolve = Clock::now();
+  std::vector<int> restartBase(K, 0);
+  PackedOutcome r;
+  if (o.precision == PDHG_FP64) r = run_packed<double>(d, o, o.max_iterations, t0, false);
+  else r = run_packed<float>(d, o, o.max_iterations, t0, o.precision == PDHG_FP32_AUTO);
+  write_results(d, r, P, 0, restartBase, results);
+  long long iterations = r.it, launches = r.launches, graphLaunches = r.graphLaunches;
+  double kernel = r.kernel;
+  long long switchAt = -1;
+  int restartsTotal = 0;
+  for (int q = 0; q < K; ++q) restartsTotal += r.restarts[q];
+  if (r.stalled) {
+    // The live instances continue with FP64 iterates from their FP32 point and
+    // primal weight; the move is a restart to that point (the anchor of the
+    // new run is where FP32 stopped).
+    switchAt = r.it;
+    if (o.progress_seconds > 0)
+      std::fprintf(stderr, "  pdhg packed: iteration %lld, FP32 progress stalled, continuing in FP64\n", r.it);
+    std::vector<int> live;
+    std::vector<std::vector<double>> xw(K), yw(K);
+    std::vector<double> ww(K, 1.0);
+    for (int q = 0; q < K; ++q) {
+      if (r.done[q]) continue;
+      const int b = d.ids[q];
+      live.push_back(b);
+      xw[b].assign(r.xs.begin() + d.colOff[q], r.xs.begin() + d.colOff[q + 1]);
+      yw[b].assign(r.ys.begin() + d.rowOff[q], r.ys.begin() + d.rowOff[q + 1]);
+      ww[b] = r.rec[q].omega;
+      restartBase[b] = r.restarts[q];
+    }
+    const PackedData d2 = pack(lps, P, live, &xw, &yw, &ww);
+    const PackedOutcome r2 = run_packed<double>(d2, o, o.max_iterations - r.it, t0, false);
+    write_results(d2, r2, P, r.it, restartBase, results);
+    iterations += r2.it;
+    launches += r2.launches;
+    graphLaunches += r2.graphLaunches;
+    kernel += r2.kernel;
+    for (size_t q = 0; q < live.size(); ++q) restartsTotal += r2.restarts[q];
+  }
+  if (stats) {
+    pdhg_stats st{};
+    st.iterations = iterations;
+    st.restarts = restartsTotal;
+    st.setup_seconds = setup;
+    st.solve_seconds = std::chrono::duration<double>(Clock::now() - tSolve).count();
+    st.kernel_seconds = kernel;
+    st.kernel_launches = launches;
+    st.graph_launches = graphLaunches;
+    st.batch = K;
+    st.precision = o.precision;
+    st.fp64_switch_iteration = switchAt;
+    std::snprintf(st.backend, sizeof(st.backend), "hip-gfx950-packed");
+    std::snprintf(st.device_name, sizeof(st.device_name), "%s", device_info(o.device).name.c_str());
+    *stats = st;
+  }
+}
+
+}  // namespace
+
+void solve_packed(const std::vector<Model>& lps, const pdhg_options& o, pdhg_result* results, pdhg_stats* stats) {
+  PDHG_REQUIRE(!lps.empty(), "no LPs");
+  PDHG_REQUIRE(o.use_gpu, "use_gpu = 0: this engine has no CPU backend");
+  PDHG_REQUIRE(o.check_every >= 2 && o.max_iterations >= 1, "bad iteration options");
+  PDHG_REQUIRE(o.precision == PDHG_FP32 || o.precision == PDHG_FP64 || o.precision == PDHG_FP32_AUTO,
+               "unknown precision");
+  const DeviceInfo& di = device_info(o.device);
+  if (!di.available) throw Error("GPU unavailable: " + di.reason);
+  solve_packed_all(lps, o, results, stats);
+}
+
+}  // namespace pdhgamd

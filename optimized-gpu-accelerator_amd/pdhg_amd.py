@@ -110,6 +110,8 @@ def lib() -> C.CDLL:
         L.pdhg_solve.argtypes = [C.POINTER(LpStruct), C.POINTER(Options), C.POINTER(ResultStruct), C.POINTER(Stats)]
         L.pdhg_solve_batch.argtypes = [C.POINTER(LpStruct), C.POINTER(CaseStruct), C.c_int, C.POINTER(Options),
                                        C.POINTER(ResultStruct), C.POINTER(Stats)]
+        L.pdhg_solve_packed.argtypes = [C.POINTER(LpStruct), C.c_int, C.POINTER(Options), C.POINTER(ResultStruct),
+                                        C.POINTER(Stats)]
         L.pdhg_prepare.argtypes = [C.POINTER(LpStruct), C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _ip, _ip, _dp]
         L.pdhg_safe_dual_bound.argtypes = [C.POINTER(LpStruct), _dp, _dp, _ip, _dp]
         L.pdhg_certify.argtypes = [C.POINTER(LpStruct), _dp, _dp, C.c_double, _dp]
@@ -557,3 +559,35 @@ class Mlp:
         losses = np.zeros(steps)
         _check(lib().pdhg_mlp_train(self.h, _d(X), _d(T), X.shape[0], steps, lr, _d(losses)), "mlp_train")
         return losses
+
+
+def solve_packed(lps: list, opts: Options | None = None):
+    """Unrelated LPs solved together by the persistent packed kernels: (results, stats)."""
+    opts = opts or default_options()
+    k = len(lps)
+    arr = (LpStruct * k)()
+    keep = []
+    for b, lp in enumerate(lps):
+        s = lp.struct()
+        keep.append(s)
+        arr[b] = s
+    xs = [np.zeros(lp.n) for lp in lps]
+    ys = [np.zeros(lp.m) for lp in lps]
+    res = (ResultStruct * k)()
+    for b in range(k):
+        res[b].x, res[b].row_dual = _d(xs[b]), _d(ys[b])
+    st = Stats()
+    _check(lib().pdhg_solve_packed(arr, k, C.byref(opts), res, C.byref(st)), "pdhg_solve_packed")
+    return [_result(res[b], xs[b], ys[b]) for b in range(k)], stats_dict(st)
+
+
+def gen_uneven_batch(seed: int, count: int, nnz_lo: float = 2e3, nnz_hi: float = 2e5) -> list:
+    """`count` random LPs of the 1000 x 2000-style generator (m = n / 2, 4 entries per
+    column) with nonzero counts log-uniform in [nnz_lo, nnz_hi]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(count):
+        nnz = float(np.exp(rng.uniform(np.log(nnz_lo), np.log(nnz_hi))))
+        n = max(8, int(round(nnz / 4)))
+        out.append(gen_random_lp(seed * 100003 + b, max(4, n // 2), n, 4.0, 0.0))
+    return out

