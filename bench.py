@@ -4,19 +4,25 @@ Workload (one "step"): the 64-instance parametric family of the refinery-style
 LP (12 periods x 40 streams x 36 products x 30 qualities, 555,852 nonzeros,
 rows/columns scaled by 10^U(-3,3); costs c_k = c0 + theta_k d on four crude
 purchase prices, theta_k ~ U[-0.3, 0.3]^4) solved as ONE batched run to
-relative KKT <= 1e-6, then every answer certified (implied bounds + rigorous
-safe dual bound, acceptance |obj - LB| <= 1e-4 (1 + |obj|)).  The value is
-certified solves per second over the timed steps (inputs built and the
-library loaded before the timer starts; each step uploads, solves and
-certifies).
+relative KKT <= 1e-6 from a learned warm start, then every answer certified
+(implied bounds + rigorous safe dual bound, acceptance
+|obj - LB| <= 1e-4 (1 + |obj|)).  A step is: MLP inference theta -> POD
+coefficients (BF16 MFMA), BF16-MFMA decode of the starting points, upload,
+batched solve, download, certification.  The value is certified solves per
+second over the timed steps.
 
-Also reported: plain single-instance GPU PDHG on a sample of the same
-instances (solves/s), the per-launch roofline of the dominant iteration
-kernel (algorithmic bytes / HIP-event time, HBM peak 8 TB/s), and the
-reference's own 16-thread CPU PDHG (compiled from the reference sources into
-oracle/_ref) timed on a bounded sample of the same batched workload.
+Before the timer: the warm-start model is fitted once on a separate sample of
+the family (PDHG solutions of other theta at a looser tolerance, POD basis,
+MLP trained on the GPU); its cost is reported as warm_start_fit_seconds.
 
-    python bench.py [--steps 2] [--warmup 1] [--gpus 1]
+Also reported: the same batched solve from a cold start, plain
+single-instance GPU PDHG on a sample of the same instances (solves/s), the
+per-launch roofline of the dominant iteration kernel (algorithmic bytes /
+HIP-event time, HBM peak 8 TB/s), and the reference's own 16-thread CPU PDHG
+(compiled from the reference sources into oracle/_ref) timed on a bounded
+sample of the same batched workload.
+
+    python bench.py [--steps 1] [--warmup 0] [--gpus 1] [--cold]
 """
 import argparse
 import json
@@ -30,17 +36,39 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "optimized-gpu-accelerator_amd"))
 
 import pdhg_amd as P  # noqa: E402
+import warmstart as W  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
-SEED_LP, SEED_THETA, K = 3, 4, 64
+SEED_LP, SEED_THETA, SEED_TRAIN, K = 3, 4, 5, 64
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def family_step(lp, costs, opts, threads=16):
-    res, st, _ = P.solve_batch(lp, costs, opts)
+def fit_warm_start(lp, n_train, tol, opts, steps):
+    """PDHG solutions of n_train other family parameters (batches of 64 at
+    tolerance tol), POD basis and GPU-trained MLP."""
+    th, costs = P.gen_family(lp, SEED_TRAIN, n_train, 0.3)
+    o = P.default_options(tolerance=tol, precision=opts.precision, max_iterations=opts.max_iterations,
+                          progress_seconds=15.0)
+    xs, ys = [], []
+    for s in range(0, n_train, 64):
+        res, st, _ = P.solve_batch(lp, costs[s:s + 64], o)
+        xs += [r.x for r in res]
+        ys += [r.y for r in res]
+        log(f"warm-start snapshots {s + len(res)}/{n_train}: {st['iterations']} iterations")
+    ws = W.WarmStart(lp, rank=32, hidden=128).fit(th, np.stack(xs), np.stack(ys), steps=steps, lr=1e-3)
+    log(f"warm-start MLP: loss {ws.losses[0]:.3g} -> {ws.losses[-1]:.3g}, POD energy kept "
+        f"{ws.energy[:ws.used].sum() / max(ws.energy.sum(), 1e-300):.6f}")
+    return ws
+
+
+def family_step(lp, thetas, costs, opts, ws=None, threads=16):
+    x0 = y0 = None
+    if ws is not None:
+        x0, y0 = ws.predict(thetas)
+    res, st, _ = P.solve_batch(lp, costs, opts, x0=x0, y0=y0)
     xs = np.stack([r.x for r in res])
     ys = np.stack([r.y for r in res])
     obj = np.array([r.objective for r in res])
@@ -85,7 +113,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp64")
+    ap.add_argument("--precision", choices=["fp32", "fp64", "fp32auto"], default="fp64")
+    ap.add_argument("--cold", action="store_true", help="no learned warm start (batched cold start only)")
+    ap.add_argument("--train", type=int, default=128, help="family parameters solved to fit the warm start")
+    ap.add_argument("--train-tol", type=float, default=1e-4, help="tolerance of the training solves")
+    ap.add_argument("--train-steps", type=int, default=3000, help="Adam steps of the MLP")
+    ap.add_argument("--no-cold-batch", action="store_true", help="skip the cold-start batched comparison run")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--plain-sample", type=int, default=2, help="instances solved one by one for the plain baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -120,25 +153,33 @@ def main():
         hip.hipFree(src)
         hip.hipFree(dst)
         log(f"calibration copy: {args.copy_calibration} MiB device to device")
-    prec = P.FP64 if args.precision == "fp64" else P.FP32
+    prec = {"fp64": P.FP64, "fp32": P.FP32, "fp32auto": P.FP32_AUTO}[args.precision]
     lp = P.gen_refinery(SEED_LP)
     thetas, costs = P.gen_family(lp, SEED_THETA, K, 0.3)
     opts = P.default_options(tolerance=args.tol, precision=prec, max_iterations=args.max_iterations,
                              progress_seconds=15.0)
     log(f"workload: refinery m={lp.m} n={lp.n} nnz={lp.nnz}, {K} instances, precision {args.precision}")
 
+    ws, fit_s = None, 0.0
+    if not args.cold:
+        t = time.perf_counter()
+        ws = fit_warm_start(lp, args.train, args.train_tol, opts, args.train_steps)
+        fit_s = time.perf_counter() - t
+        log(f"warm-start fit: {fit_s:.1f} s ({args.train} parameters at tolerance {args.train_tol})")
+
     for w in range(args.warmup):
         t = time.perf_counter()
-        res, st, cert, ok = family_step(lp, costs, opts)
+        res, st, cert, ok = family_step(lp, thetas, costs, opts, ws)
         log(f"warmup {w}: {time.perf_counter() - t:.2f} s, {st['iterations']} iterations, {ok.sum()}/{K} certified")
 
-    certified, iters, kkt, gaps = 0, [], [], []
+    certified, iters, kkt, gaps, inst_iters = 0, [], [], [], []
     t0 = time.perf_counter()
     for s in range(args.steps):
         ts = time.perf_counter()
-        res, st, cert, ok = family_step(lp, costs, opts)
+        res, st, cert, ok = family_step(lp, thetas, costs, opts, ws)
         certified += int(ok.sum())
         iters.append(st["iterations"])
+        inst_iters += [r.iterations for r in res]
         kkt.append(max(r.relative_kkt for r in res))
         gaps.append(float(cert[:, 1].max()))
         log(f"step {s}: {time.perf_counter() - ts:.2f} s, {st['iterations']} iterations, {ok.sum()}/{K} certified, "
@@ -146,6 +187,16 @@ def main():
     elapsed = time.perf_counter() - t0
     value = certified / elapsed
     gpu_iters = int(max(iters))
+
+    # The same batched solve from a cold start (one run, for the ratio).
+    cold = None
+    if ws is not None and not args.no_cold_batch:
+        t = time.perf_counter()
+        res_c, st_c, _, ok_c = family_step(lp, thetas, costs, opts, None)
+        el = time.perf_counter() - t
+        cold = {"solves_per_s": int(ok_c.sum()) / el, "seconds": el, "iterations": st_c["iterations"],
+                "median_instance_iterations": float(np.median([r.iterations for r in res_c]))}
+        log(f"batched cold start: {el:.2f} s, {st_c['iterations']} iterations, {ok_c.sum()}/{K} certified")
 
     # Kernel roofline: HIP events on the run's own stream, same workload shape.
     _, kst, _ = P.solve_batch(lp, costs, P.default_options(tolerance=args.tol, precision=prec, max_iterations=64 * 8,
@@ -155,14 +206,15 @@ def main():
     dom = max(kern, key=lambda k: kern[k][1])
     dbytes, dms = kern[dom]
     achieved = dbytes / (dms * 1e-3) / 1e9
+    kprec = "fp64" if prec == P.FP64 else "fp32"
     traffic = None
     tf = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get(f"{dom}_{args.precision}")
+        traffic = json.load(open(tf)).get(f"{dom}_{kprec}")
     log(f"kernels: k_col {kern['k_col'][1] * 1e3:.1f} us ({kern['k_col'][0] / kern['k_col'][1] / 1e6:.0f} GB/s), "
         f"k_row {kern['k_row'][1] * 1e3:.1f} us ({kern['k_row'][0] / kern['k_row'][1] / 1e6:.0f} GB/s)")
 
-    # Plain single-instance GPU PDHG on a sample of the same instances.
+    # Plain single-instance GPU PDHG (cold start) on a sample of the same instances.
     tp = time.perf_counter()
     plain_ok = 0
     plain_iters = []
@@ -178,8 +230,9 @@ def main():
     if cpu:
         log(f"cpu baseline: {cpu['value']:.4f} solves/s ({cpu['sample']})")
 
+    mode = "batched+warm" if ws is not None else "batched"
     out = {
-        "metric": "LP solves/s at rel-KKT<=1e-6 (batched 64-instance family, certified)",
+        "metric": f"LP solves/s at rel-KKT<=1e-6 ({mode} 64-instance family, certified)",
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
@@ -189,21 +242,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == P.FP64 else "f32",
+        "dtype": {P.FP64: "f64", P.FP32: "f32", P.FP32_AUTO: "f32+f64"}[prec],
         "data": "synthetic",
         "config": {"workload": f"refinery-family-{K}: m={lp.m} n={lp.n} nnz={lp.nnz}, {K} cost cases, tol {args.tol}",
-                   "batch": K, "iterations": gpu_iters, "certified": certified, "solved": args.steps * K},
+                   "batch": K, "start": "learned warm start" if ws is not None else "cold",
+                   "iterations": gpu_iters, "median_instance_iterations": float(np.median(inst_iters)),
+                   "certified": certified, "solved": args.steps * K},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "algorithmic_bytes": dbytes,
                      "launch_us": dms * 1e3},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "plain_gpu_solves_per_s": plain,
-        "batched_vs_plain": (value / plain) if plain else None,
+        "vs_plain_gpu": (value / plain) if plain else None,
+        "batched_cold": cold,
+        "warm_start_fit_seconds": fit_s if ws is not None else None,
         "max_rel_kkt": max(kkt),
         "max_certified_gap": max(gaps),
     }
     print(json.dumps(out))
-
 
 if __name__ == "__main__":
     main()

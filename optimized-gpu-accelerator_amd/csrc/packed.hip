@@ -37,28 +37,51 @@ struct PScal {  // per-instance scalars
   const double *eta, *beta, *gamma, *offset;
 };
 
-// Draws the next task ticket for the whole workgroup (-1: queue empty).
-__device__ inline int next_ticket(int* queue, int ntasks) {
+// Rows of one task: up to kTaskGroups workgroup-sized groups of one bucket of
+// one instance, processed one group after the other.
+constexpr int kTaskGroups = 4;
+
+// Eight work queues, one per XCD: workgroups are dispatched round-robin over
+// the XCDs, so queue q (blockIdx % 8 == q) serves one XCD and covers the
+// contiguous task range beg[q] .. beg[q+1] (neighbouring instances share that
+// XCD's L2).  Each queue's counter has its own 128-byte line.
+struct Queues {
+  int beg[9];
+};
+constexpr int kQueueStride = 32;  // ints between counters
+constexpr int kQueueSet = 8 * kQueueStride;
+
+// Draws the next task for the whole workgroup from its queue (-1: empty).
+// Every workgroup of the grid draws exactly one exhausted ticket per launch,
+// so the exhausted tickets of queue q run from len to len + nwg - 1 and the
+// holder of the last one resets the counter for the next launch.
+__device__ inline int next_ticket(int* counters, const Queues& qs) {
   __shared__ int ticket;
+  const int q = blockIdx.x & 7;
+  const int len = qs.beg[q + 1] - qs.beg[q];
+  const int nwg = (static_cast<int>(gridDim.x) - q + 7) >> 3;
   __syncthreads();  // previous iteration's readers are done
   if (threadIdx.x == 0) {
-    const int t = atomicAdd(queue, 1);
-    // Exhausted tickets run from ntasks to ntasks + gridDim.x - 1, one per
-    // workgroup; the holder of the last one resets the queue.
-    if (t == ntasks + static_cast<int>(gridDim.x) - 1) atomicExch(queue, 0);
-    ticket = t;
+    int* ctr = counters + q * kQueueStride;
+    const int t = atomicAdd(ctr, 1);
+    if (t == len + nwg - 1) atomicExch(ctr, 0);
+    ticket = t < len ? qs.beg[q] + t : -1;
   }
   __syncthreads();
-  return ticket < ntasks ? ticket : -1;
+  return ticket;
 }
 
 struct PSlot {
   int row, sub, logS;
 };
-__device__ inline PSlot task_slot(const PTask& t, const int* perm) {
+__device__ inline int task_groups(const PTask& t) {
+  const int per = kWG >> t.logS;
+  return (t.count + per - 1) / per;
+}
+__device__ inline PSlot task_slot(const PTask& t, const int* perm, int g) {
   PSlot s;
   s.logS = t.logS;
-  const int r = threadIdx.x >> t.logS;
+  const int r = g * (kWG >> t.logS) + (threadIdx.x >> t.logS);
   s.sub = threadIdx.x & ((1 << t.logS) - 1);
   s.row = r < t.count ? perm[t.start + r] : -1;
   return s;
@@ -66,40 +89,44 @@ __device__ inline PSlot task_slot(const PTask& t, const int* perm) {
 
 template <typename T, bool CHECK>
 __global__ __launch_bounds__(kWG) void k_pcol(const PTask* __restrict__ tasks, int ntasks, const int* __restrict__ perm,
-                                              int* queue, const int* __restrict__ ptr, const int* __restrict__ idx,
-                                              const T* __restrict__ val, const T* __restrict__ y,
-                                              const T* __restrict__ yp, T* __restrict__ x, T* __restrict__ xp,
-                                              T* __restrict__ x0, const T* __restrict__ c, const T* __restrict__ lo,
-                                              const T* __restrict__ up, Inst st, int tloc, double rho,
-                                              double* __restrict__ part) {
-  for (int tk; (tk = next_ticket(queue, ntasks)) >= 0;) {
+                                              int* queue, Queues qs, const int* __restrict__ ptr,
+                                              const int* __restrict__ idx, const T* __restrict__ val,
+                                              const T* __restrict__ y, const T* __restrict__ yp, T* __restrict__ x,
+                                              T* __restrict__ xp, T* __restrict__ x0, const T* __restrict__ c,
+                                              const T* __restrict__ lo, const T* __restrict__ up, Inst st, int tloc,
+                                              double rho, double* __restrict__ part) {
+  for (int tk; (tk = next_ticket(queue, qs)) >= 0;) {
     const PTask task = tasks[tk];
     const int b = task.inst;
     if (st.done[b]) continue;
-    const PSlot s = task_slot(task, perm);
     const long long it = st.iter[b] + tloc;
     const bool rst = st.applyAt[b] == it;
     const T* __restrict__ ysrc = rst ? yp : y;
-    T acc = 0;
-    if (s.row >= 0) acc = row_dot<0, T>(idx, val, ysrc, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, 0);
-    acc = reduce_sub<0>(acc, s.logS);
+    const T tau = static_cast<T>(st.tau[b]);
+    const double kk = static_cast<double>(it - st.restartAt[b]);
+    const double lam = (kk + 1.0) / (kk + 2.0);
     double q[2] = {0.0, 0.0};
-    if (s.row >= 0 && s.sub == 0) {
-      const int j = s.row;
-      const T xin = rst ? xp[j] : x[j];
-      const T xo = rst ? xin : x0[j];
-      if (rst) x0[j] = xin;
-      const T v = clamp_to<T>(xin - static_cast<T>(st.tau[b]) * (c[j] - acc), lo[j], up[j]);
-      const double kk = static_cast<double>(it - st.restartAt[b]);
-      const double lam = (kk + 1.0) / (kk + 2.0);
-      x[j] = static_cast<T>(lam) * (static_cast<T>(1.0 + rho) * v - static_cast<T>(rho) * xin) +
-             static_cast<T>(1.0 - lam) * xo;
-      xp[j] = v;
-      if (CHECK) {
-        const double d = static_cast<double>(xin) - static_cast<double>(v);
-        const double e = static_cast<double>(v) - static_cast<double>(xo);
-        q[0] = d * d;
-        q[1] = e * e;
+    const int ng = task_groups(task);
+    for (int g = 0; g < ng; ++g) {
+      const PSlot s = task_slot(task, perm, g);
+      T acc = 0;
+      if (s.row >= 0) acc = row_dot<0, T>(idx, val, ysrc, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, 0);
+      acc = reduce_sub<0>(acc, s.logS);
+      if (s.row >= 0 && s.sub == 0) {
+        const int j = s.row;
+        const T xin = rst ? xp[j] : x[j];
+        const T xo = rst ? xin : x0[j];
+        if (rst) x0[j] = xin;
+        const T v = clamp_to<T>(xin - tau * (c[j] - acc), lo[j], up[j]);
+        x[j] = static_cast<T>(lam) * (static_cast<T>(1.0 + rho) * v - static_cast<T>(rho) * xin) +
+               static_cast<T>(1.0 - lam) * xo;
+        xp[j] = v;
+        if (CHECK) {
+          const double d = static_cast<double>(xin) - static_cast<double>(v);
+          const double e = static_cast<double>(v) - static_cast<double>(xo);
+          q[0] += d * d;
+          q[1] += e * e;
+        }
       }
     }
     if (CHECK) block_partials<0, 2>(q, part, 0, ntasks, tk);
@@ -108,65 +135,69 @@ __global__ __launch_bounds__(kWG) void k_pcol(const PTask* __restrict__ tasks, i
 
 template <typename T, bool CHECK>
 __global__ __launch_bounds__(kWG) void k_prow(const PTask* __restrict__ tasks, int ntasks, const int* __restrict__ perm,
-                                              int* queue, const int* __restrict__ ptr, const int* __restrict__ idx,
-                                              const T* __restrict__ val, const double* __restrict__ val64,
-                                              const T* __restrict__ xp, T* __restrict__ y, T* __restrict__ yp,
-                                              T* __restrict__ y0, T* __restrict__ ax, T* __restrict__ axp,
-                                              T* __restrict__ ax0, const T* __restrict__ rl, const T* __restrict__ ru,
+                                              int* queue, Queues qs, const int* __restrict__ ptr,
+                                              const int* __restrict__ idx, const T* __restrict__ val,
+                                              const double* __restrict__ val64, const T* __restrict__ xp,
+                                              T* __restrict__ y, T* __restrict__ yp, T* __restrict__ y0,
+                                              T* __restrict__ ax, T* __restrict__ axp, T* __restrict__ ax0,
+                                              const T* __restrict__ rl, const T* __restrict__ ru,
                                               const double* __restrict__ rl64, const double* __restrict__ ru64,
                                               const double* __restrict__ rfac, Inst st, int tloc, double rho,
                                               double* __restrict__ part) {
-  for (int tk; (tk = next_ticket(queue, ntasks)) >= 0;) {
+  for (int tk; (tk = next_ticket(queue, qs)) >= 0;) {
     const PTask task = tasks[tk];
     const int b = task.inst;
     if (st.done[b]) continue;
-    const PSlot s = task_slot(task, perm);
-    T acc = 0;
-    double acc64 = 0;
-    if (s.row >= 0) {
-      const int e0 = ptr[s.row] + s.sub, e1 = ptr[s.row + 1], step = 1 << s.logS;
-      acc = row_dot<0, T>(idx, val, xp, e0, e1, step, 0);
-      if (CHECK) acc64 = row_dot<0, double>(idx, val64, xp, e0, e1, step, 0);
-    }
-    acc = reduce_sub<0>(acc, s.logS);
-    if (CHECK) acc64 = reduce_sub<0>(acc64, s.logS);
+    const long long it = st.iter[b] + tloc;
+    const bool rst = st.applyAt[b] == it;
+    const T sg = static_cast<T>(st.sigma[b]);
+    const double kk = static_cast<double>(it - st.restartAt[b]);
+    const double lam = (kk + 1.0) / (kk + 2.0);
+    const T lamT = static_cast<T>(lam), omT = static_cast<T>(1.0 - lam);
+    const T r1 = static_cast<T>(1.0 + rho), r0 = static_cast<T>(rho);
     double q[4] = {0.0, 0.0, 0.0, 0.0};
-    if (s.row >= 0 && s.sub == 0) {
-      const int i = s.row;
-      const long long it = st.iter[b] + tloc;
-      const bool rst = st.applyAt[b] == it;
-      const T yin = rst ? yp[i] : y[i];
-      const T axin = rst ? axp[i] : ax[i];
-      const T yo = rst ? yin : y0[i];
-      const T axo = rst ? axin : ax0[i];
-      if (rst) {
-        y0[i] = yin;
-        ax0[i] = axin;
+    const int ng = task_groups(task);
+    for (int g = 0; g < ng; ++g) {
+      const PSlot s = task_slot(task, perm, g);
+      T acc = 0;
+      double acc64 = 0;
+      if (s.row >= 0) {
+        const int e0 = ptr[s.row] + s.sub, e1 = ptr[s.row + 1], step = 1 << s.logS;
+        acc = row_dot<0, T>(idx, val, xp, e0, e1, step, 0);
+        if (CHECK) acc64 = row_dot<0, double>(idx, val64, xp, e0, e1, step, 0);
       }
-      // Reflected dual step in the cancellation-free form y+ = y - sigma (g - w).
-      const T sg = static_cast<T>(st.sigma[b]);
-      const T g = static_cast<T>(2) * acc - axin;
-      const T tq = yin / sg;
-      const T dlo = g - rl[i], dhi = g - ru[i];
-      const T yplus = dlo < tq ? yin - sg * dlo : (dhi > tq ? yin - sg * dhi : static_cast<T>(0));
-      const double kk = static_cast<double>(it - st.restartAt[b]);
-      const double lam = (kk + 1.0) / (kk + 2.0);
-      const T lamT = static_cast<T>(lam), omT = static_cast<T>(1.0 - lam);
-      const T r1 = static_cast<T>(1.0 + rho), r0 = static_cast<T>(rho);
-      y[i] = lamT * (r1 * yplus - r0 * yin) + omT * yo;
-      ax[i] = lamT * (r1 * acc - r0 * axin) + omT * axo;
-      if (CHECK) {
-        yp[i] = yplus;
-        axp[i] = acc;
-        const double L = rl64[i], U = ru64[i];
-        const double res = (fmax(L - acc64, 0.0) + fmax(acc64 - U, 0.0)) * rfac[i];
-        const double yy = static_cast<double>(yplus);
-        q[0] = res * res;
-        if (yy > 0 && L > -1e300) q[1] = yy * L;
-        else if (yy < 0 && U < 1e300) q[1] = yy * U;
-        const double d = static_cast<double>(yin) - yy, e = yy - static_cast<double>(yo);
-        q[2] = d * d;
-        q[3] = e * e;
+      acc = reduce_sub<0>(acc, s.logS);
+      if (CHECK) acc64 = reduce_sub<0>(acc64, s.logS);
+      if (s.row >= 0 && s.sub == 0) {
+        const int i = s.row;
+        const T yin = rst ? yp[i] : y[i];
+        const T axin = rst ? axp[i] : ax[i];
+        const T yo = rst ? yin : y0[i];
+        const T axo = rst ? axin : ax0[i];
+        if (rst) {
+          y0[i] = yin;
+          ax0[i] = axin;
+        }
+        // Reflected dual step in the cancellation-free form y+ = y - sigma (g - w).
+        const T gr = static_cast<T>(2) * acc - axin;
+        const T tq = yin / sg;
+        const T dlo = gr - rl[i], dhi = gr - ru[i];
+        const T yplus = dlo < tq ? yin - sg * dlo : (dhi > tq ? yin - sg * dhi : static_cast<T>(0));
+        y[i] = lamT * (r1 * yplus - r0 * yin) + omT * yo;
+        ax[i] = lamT * (r1 * acc - r0 * axin) + omT * axo;
+        if (CHECK) {
+          yp[i] = yplus;
+          axp[i] = acc;
+          const double L = rl64[i], U = ru64[i];
+          const double res = (fmax(L - acc64, 0.0) + fmax(acc64 - U, 0.0)) * rfac[i];
+          const double yy = static_cast<double>(yplus);
+          q[0] += res * res;
+          if (yy > 0 && L > -1e300) q[1] += yy * L;
+          else if (yy < 0 && U < 1e300) q[1] += yy * U;
+          const double d = static_cast<double>(yin) - yy, e = yy - static_cast<double>(yo);
+          q[2] += d * d;
+          q[3] += e * e;
+        }
       }
     }
     if (CHECK) block_partials<0, 4>(q, part, 0, ntasks, tk);
@@ -175,35 +206,38 @@ __global__ __launch_bounds__(kWG) void k_prow(const PTask* __restrict__ tasks, i
 
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_pcolcheck(const PTask* __restrict__ tasks, int ntasks,
-                                                   const int* __restrict__ perm, int* queue,
+                                                   const int* __restrict__ perm, int* queue, Queues qs,
                                                    const int* __restrict__ ptr, const int* __restrict__ idx,
                                                    const double* __restrict__ val64, const T* __restrict__ yp,
                                                    const T* __restrict__ xp, const double* __restrict__ c64,
                                                    const double* __restrict__ lo64, const double* __restrict__ up64,
                                                    const double* __restrict__ cfac, Inst st,
                                                    double* __restrict__ part) {
-  for (int tk; (tk = next_ticket(queue, ntasks)) >= 0;) {
+  for (int tk; (tk = next_ticket(queue, qs)) >= 0;) {
     const PTask task = tasks[tk];
     if (st.done[task.inst]) continue;
-    const PSlot s = task_slot(task, perm);
-    double acc = 0;
-    if (s.row >= 0) acc = row_dot<0, double>(idx, val64, yp, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, 0);
-    acc = reduce_sub<0>(acc, s.logS);
     double q[3] = {0.0, 0.0, 0.0};
-    if (s.row >= 0 && s.sub == 0) {
-      const int j = s.row;
-      const double cj = c64[j], l = lo64[j], u = up64[j];
-      const double d = cj - acc;
-      const bool hl = l > -1e300, hu = u < 1e300;
-      double res = 0;
-      if (!hl && !hu) res = fabs(d);
-      else if (!hl) res = fmax(d, 0.0);
-      else if (!hu) res = fmax(-d, 0.0);
-      res *= cfac[j];
-      q[0] = res * res;
-      q[1] = cj * static_cast<double>(xp[j]);
-      if (d > 0 && hl) q[2] = d * l;
-      else if (d < 0 && hu) q[2] = d * u;
+    const int ng = task_groups(task);
+    for (int g = 0; g < ng; ++g) {
+      const PSlot s = task_slot(task, perm, g);
+      double acc = 0;
+      if (s.row >= 0) acc = row_dot<0, double>(idx, val64, yp, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, 0);
+      acc = reduce_sub<0>(acc, s.logS);
+      if (s.row >= 0 && s.sub == 0) {
+        const int j = s.row;
+        const double cj = c64[j], l = lo64[j], u = up64[j];
+        const double d = cj - acc;
+        const bool hl = l > -1e300, hu = u < 1e300;
+        double res = 0;
+        if (!hl && !hu) res = fabs(d);
+        else if (!hl) res = fmax(d, 0.0);
+        else if (!hu) res = fmax(-d, 0.0);
+        res *= cfac[j];
+        q[0] += res * res;
+        q[1] += cj * static_cast<double>(xp[j]);
+        if (d > 0 && hl) q[2] += d * l;
+        else if (d < 0 && hu) q[2] += d * u;
+      }
     }
     block_partials<0, 3>(q, part, 2, ntasks, tk);
   }
@@ -256,21 +290,26 @@ __global__ __launch_bounds__(kWG) void k_pfinalize(Inst st, const double* __rest
 // out = A in over all tasks (initial activities).
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_pspmv(const PTask* __restrict__ tasks, int ntasks, const int* __restrict__ perm,
-                                               int* queue, const int* __restrict__ ptr, const int* __restrict__ idx,
-                                               const T* __restrict__ val, const T* __restrict__ in, T* __restrict__ out) {
-  for (int tk; (tk = next_ticket(queue, ntasks)) >= 0;) {
-    const PSlot s = task_slot(tasks[tk], perm);
-    T acc = 0;
-    if (s.row >= 0) acc = row_dot<0, T>(idx, val, in, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, 0);
-    acc = reduce_sub<0>(acc, s.logS);
-    if (s.row >= 0 && s.sub == 0) out[s.row] = acc;
+                                               int* queue, Queues qs, const int* __restrict__ ptr,
+                                               const int* __restrict__ idx, const T* __restrict__ val,
+                                               const T* __restrict__ in, T* __restrict__ out) {
+  for (int tk; (tk = next_ticket(queue, qs)) >= 0;) {
+    const PTask task = tasks[tk];
+    const int ng = task_groups(task);
+    for (int g = 0; g < ng; ++g) {
+      const PSlot s = task_slot(task, perm, g);
+      T acc = 0;
+      if (s.row >= 0) acc = row_dot<0, T>(idx, val, in, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, 0);
+      acc = reduce_sub<0>(acc, s.logS);
+      if (s.row >= 0 && s.sub == 0) out[s.row] = acc;
+    }
   }
 }
 
 // Tasks of a block-diagonal CSR: per instance, rows bucketed by the lane count
-// nextpow2(ceil(len / nnzPerLane)) (capped at 64), each task one workgroup's
-// worth of rows of one bucket of one instance.  beg[b] .. beg[b+1] are the
-// tasks of instance b.
+// nextpow2(ceil(len / nnzPerLane)) (capped at 64), each task up to
+// kTaskGroups workgroups' worth of rows of one bucket of one instance.
+// beg[b] .. beg[b+1] are the tasks of instance b.
 void build_tasks(const std::vector<int>& ptr, const std::vector<int>& rowOff, int nnzPerLane, std::vector<PTask>& tasks,
                  std::vector<int>& perm, std::vector<int>& beg) {
   const int K = static_cast<int>(rowOff.size()) - 1;
@@ -286,7 +325,7 @@ void build_tasks(const std::vector<int>& ptr, const std::vector<int>& rowOff, in
       bucket[std::min(6, ilog2(std::max(1, want)))].push_back(r);
     }
     for (int l = 0; l <= 6; ++l) {
-      const int per = kWG >> l;
+      const int per = (kWG >> l) * kTaskGroups;
       for (size_t a = 0; a < bucket[l].size(); a += per) {
         PTask t;
         t.inst = b;
@@ -299,6 +338,24 @@ void build_tasks(const std::vector<int>& ptr, const std::vector<int>& rowOff, in
     }
   }
   beg[K] = static_cast<int>(tasks.size());
+}
+
+// The live part of a task list: tasks of instances not done, in order, with
+// their per-instance ranges (empty for finished instances) and the split of
+// the list over the eight queues.
+void live_tasks(const std::vector<PTask>& all, const std::vector<int>& allBeg, const std::vector<int>& done,
+                std::vector<PTask>& out, std::vector<int>& beg, Queues& qs) {
+  const int K = static_cast<int>(allBeg.size()) - 1;
+  out.clear();
+  beg.assign(K + 1, 0);
+  for (int b = 0; b < K; ++b) {
+    beg[b] = static_cast<int>(out.size());
+    if (!done.empty() && done[b]) continue;
+    out.insert(out.end(), all.begin() + allBeg[b], all.begin() + allBeg[b + 1]);
+  }
+  beg[K] = static_cast<int>(out.size());
+  const int n = beg[K];
+  for (int q = 0; q <= 8; ++q) qs.beg[q] = static_cast<int>((static_cast<long long>(n) * q) / 8);
 }
 
 template <typename T>
@@ -316,19 +373,18 @@ class PackedRun {
     HIP_OK(hipEventCreate(&e1_));
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, device));
-    grid_ = std::max(1, prop.multiProcessorCount) * 4;
-    std::vector<PTask> ct, rt;
+    // Four workgroups per CU, a multiple of 8 so every queue has workgroups.
+    grid_ = std::max(8, (std::max(1, prop.multiProcessorCount) * 4 + 7) / 8 * 8);
     std::vector<int> cperm, rperm;
-    build_tasks(AT.ptr, colOff, nnzPerLane, ct, cperm, colBeg_);
-    build_tasks(A.ptr, rowOff, nnzPerLane, rt, rperm, rowBeg_);
-    nCol_ = static_cast<int>(ct.size());
-    nRow_ = static_cast<int>(rt.size());
-    colTasks_ = up(ct);
-    rowTasks_ = up(rt);
+    build_tasks(AT.ptr, colOff, nnzPerLane, colAll_, cperm, colBegAll_);
+    build_tasks(A.ptr, rowOff, nnzPerLane, rowAll_, rperm, rowBegAll_);
+    colTasks_ = up(colAll_);
+    rowTasks_ = up(rowAll_);
     colPerm_ = up(cperm);
     rowPerm_ = up(rperm);
-    dColBeg_ = up(colBeg_);
-    dRowBeg_ = up(rowBeg_);
+    dColBeg_ = up(colBegAll_);
+    dRowBeg_ = up(rowBegAll_);
+    set_tasks({});
     aPtr_ = up(A.ptr);
     aIdx_ = up(A.idx);
     tPtr_ = up(AT.ptr);
@@ -348,10 +404,11 @@ class PackedRun {
     for (T** v : {&y_, &yp_, &y0_, &ax_, &axp_, &ax0_, &rl_, &ru_}) *v = buf_.alloc<T>(M_);
     for (double** v : {&c64_, &lo64_, &up64_}) *v = buf_.alloc<double>(N_);
     for (double** v : {&rl64_, &ru64_}) *v = buf_.alloc<double>(M_);
-    colPart_ = buf_.alloc<double>(static_cast<size_t>(5) * nCol_);
-    rowPart_ = buf_.alloc<double>(static_cast<size_t>(4) * nRow_);
-    queue_ = buf_.alloc<int>(4);
-    HIP_OK(hipMemset(queue_, 0, 4 * sizeof(int)));
+    colPart_ = buf_.alloc<double>(static_cast<size_t>(5) * colAll_.size());
+    rowPart_ = buf_.alloc<double>(static_cast<size_t>(4) * rowAll_.size());
+    queue_ = buf_.alloc<int>(4 * kQueueSet);
+    std::vector<int> zq(4 * kQueueSet, 0);
+    putv(queue_, zq);
     st_.tau = buf_.alloc<double>(K_);
     st_.sigma = buf_.alloc<double>(K_);
     st_.omega = buf_.alloc<double>(K_);
@@ -429,8 +486,8 @@ class PackedRun {
   }
 
   void start() {
-    hipLaunchKernelGGL(k_pspmv<T>, dim3(grid_), dim3(kWG), 0, s_, rowTasks_, nRow_, rowPerm_, queue_ + 3, aPtr_, aIdx_,
-                       aVal_, x_, ax_);
+    hipLaunchKernelGGL(k_pspmv<T>, dim3(grid_), dim3(kWG), 0, s_, rowTasks_, nRow_, rowPerm_, queue_ + 3 * kQueueSet,
+                       rowQ_, aPtr_, aIdx_, aVal_, x_, ax_);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(ax0_, ax_, sizeof(T) * M_, hipMemcpyDeviceToDevice, s_));
     HIP_OK(hipMemcpyAsync(axp_, ax_, sizeof(T) * M_, hipMemcpyDeviceToDevice, s_));
@@ -482,23 +539,58 @@ class PackedRun {
     return ms / 1000.0;
   }
 
-  long long launches = 0, graphLaunches = 0;
+  long long launches = 0, graphLaunches = 0, taskRebuilds = 0;
   int tasks() const { return nCol_ + nRow_; }
 
+  // Drops the tasks of finished instances once they are at least half of the
+  // current lists (the kernels skip them anyway, but every skipped task still
+  // costs a ticket); the graph is captured again for the new lists.
+  void drop_finished(const std::vector<int>& done) {
+    int live = 0;
+    for (int b = 0; b < K_; ++b)
+      if (!done[b]) live += (colBegAll_[b + 1] - colBegAll_[b]) + (rowBegAll_[b + 1] - rowBegAll_[b]);
+    if (live == 0 || 2 * live > nCol_ + nRow_) return;
+    set_tasks(done);
+    if (exec_) {
+      HIP_OK(hipGraphExecDestroy(exec_));
+      exec_ = nullptr;
+    }
+    if (graph_) {
+      HIP_OK(hipGraphDestroy(graph_));
+      graph_ = nullptr;
+    }
+    ++taskRebuilds;
+  }
+
  private:
+  // Uploads the live task lists (all tasks when done is empty) into the
+  // task buffers, which were sized for the full lists.
+  void set_tasks(const std::vector<int>& done) {
+    std::vector<PTask> ct, rt;
+    std::vector<int> cb, rb;
+    live_tasks(colAll_, colBegAll_, done, ct, cb, colQ_);
+    live_tasks(rowAll_, rowBegAll_, done, rt, rb, rowQ_);
+    nCol_ = static_cast<int>(ct.size());
+    nRow_ = static_cast<int>(rt.size());
+    if (!ct.empty()) putv(colTasks_, ct);
+    if (!rt.empty()) putv(rowTasks_, rt);
+    putv(dColBeg_, cb);
+    putv(dRowBeg_, rb);
+  }
+
   void enqueue(int iters) {
     for (int t = 0; t < iters; ++t) {
       const bool chk = t == iters - 1;
       auto colK = chk ? k_pcol<T, true> : k_pcol<T, false>;
       auto rowK = chk ? k_prow<T, true> : k_prow<T, false>;
-      hipLaunchKernelGGL(colK, dim3(grid_), dim3(kWG), 0, s_, colTasks_, nCol_,
-                         colPerm_, queue_ + 0, tPtr_, tIdx_, tVal_, y_, yp_, x_, xp_, x0_, c_, lo_, up_, st_, t, rho_,
-                         colPart_);
-      hipLaunchKernelGGL(rowK, dim3(grid_), dim3(kWG), 0, s_, rowTasks_, nRow_,
-                         rowPerm_, queue_ + 1, aPtr_, aIdx_, aVal_, aVal64_, xp_, y_, yp_, y0_, ax_, axp_, ax0_, rl_,
-                         ru_, rl64_, ru64_, rfac_, st_, t, rho_, rowPart_);
+      hipLaunchKernelGGL(colK, dim3(grid_), dim3(kWG), 0, s_, colTasks_, nCol_, colPerm_, queue_, colQ_, tPtr_,
+                         tIdx_, tVal_, y_, yp_, x_, xp_, x0_, c_, lo_, up_, st_, t, rho_, colPart_);
+      hipLaunchKernelGGL(rowK, dim3(grid_), dim3(kWG), 0, s_, rowTasks_, nRow_, rowPerm_, queue_ + kQueueSet, rowQ_,
+                         aPtr_, aIdx_, aVal_, aVal64_, xp_, y_, yp_, y0_, ax_, axp_, ax0_, rl_, ru_, rl64_, ru64_,
+                         rfac_, st_, t, rho_, rowPart_);
     }
-    hipLaunchKernelGGL(k_pcolcheck<T>, dim3(grid_), dim3(kWG), 0, s_, colTasks_, nCol_, colPerm_, queue_ + 2, tPtr_,
+    hipLaunchKernelGGL(k_pcolcheck<T>, dim3(grid_), dim3(kWG), 0, s_, colTasks_, nCol_, colPerm_,
+                       queue_ + 2 * kQueueSet, colQ_, tPtr_,
                        tIdx_, tVal64_, yp_, xp_, c64_, lo64_, up64_, cfac_, st_, colPart_);
     hipLaunchKernelGGL(k_pfinalize, dim3(K_), dim3(kWG), 0, s_, st_, colPart_, nCol_, dColBeg_, rowPart_, nRow_,
                        dRowBeg_, ps_, tol_, iters, checkEvery_);
@@ -536,7 +628,9 @@ class PackedRun {
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
   Buffers buf_;
-  std::vector<int> colBeg_, rowBeg_;
+  std::vector<PTask> colAll_, rowAll_;
+  std::vector<int> colBegAll_, rowBegAll_;
+  Queues colQ_{}, rowQ_{};
   PTask *colTasks_ = nullptr, *rowTasks_ = nullptr;
   int *colPerm_ = nullptr, *rowPerm_ = nullptr, *dColBeg_ = nullptr, *dRowBeg_ = nullptr, *queue_ = nullptr;
   int *aPtr_ = nullptr, *aIdx_ = nullptr, *tPtr_ = nullptr, *tIdx_ = nullptr;
@@ -711,6 +805,7 @@ PackedOutcome run_packed(const PackedData& d, const pdhg_options& o, long long m
       r.stalled = true;
       break;
     }
+    run.drop_finished(r.done);
   }
   r.kernel = run.stop();
   run.download(true, r.xs);
