@@ -102,13 +102,46 @@ __device__ inline Acc row_dot(const int* __restrict__ idx, const V* __restrict__
   return (a0 + a1) + (a2 + a3);
 }
 
+// Vector layout of a B-lane run (B = 1 << LOGB): element (i, b) sits at
+// i*B + b, except for FP64 at B = 64 ("split"): the 64 values of row i take
+// 128 dwords, the low words of lanes 0..63 followed by their high words.  A
+// 64-lane access is then two fully coalesced dword accesses; a 64-lane
+// dwordx2 access costs the vector L1 twice the cycles per byte.
+template <typename T, int LOGB>
+constexpr bool kSplit = sizeof(T) == 8 && LOGB == 6;
+
+template <bool SPLIT, int LOGB, typename T>
+__device__ inline T vld(const T* __restrict__ p, size_t i, int b) {
+  if constexpr (SPLIT) {
+    static_assert(sizeof(T) == 8 && LOGB == 6, "split layout is FP64 at B = 64");
+    const unsigned* u = reinterpret_cast<const unsigned*>(p) + (i << 7);
+    const unsigned lo = u[b], hi = u[64 + b];
+    return __builtin_bit_cast(T, (static_cast<unsigned long long>(hi) << 32) | lo);
+  } else {
+    return p[(i << LOGB) + b];
+  }
+}
+
+template <bool SPLIT, int LOGB, typename T>
+__device__ inline void vst(T* __restrict__ p, size_t i, int b, T v) {
+  if constexpr (SPLIT) {
+    static_assert(sizeof(T) == 8 && LOGB == 6, "split layout is FP64 at B = 64");
+    unsigned* u = reinterpret_cast<unsigned*>(p) + (i << 7);
+    const unsigned long long bits = __builtin_bit_cast(unsigned long long, v);
+    u[b] = static_cast<unsigned>(bits);
+    u[64 + b] = static_cast<unsigned>(bits >> 32);
+  } else {
+    p[(i << LOGB) + b] = v;
+  }
+}
+
 // B = 64: a whole wavefront serves one row (lane = instance), so the row's
 // bounds are wave-uniform.  Column indices and values then come through the
 // scalar cache (s_load) eight at a time, and the eight 64-lane gathers of
 // a chunk are issued back to back before the first multiply-add waits.  The
 // last partial chunk is predicated: its spare slots re-read the row's first
 // column (a cache hit) with a zero weight, so no serial tail remains.
-template <typename Acc, typename V, typename X>
+template <typename Acc, bool SPLIT, typename V, typename X>
 __device__ inline Acc row_dot_wave(const int* __restrict__ idx, const V* __restrict__ val, const X* __restrict__ x,
                                    int e0, int e1, int b) {
   constexpr int CH = 8;
@@ -125,7 +158,7 @@ __device__ inline Acc row_dot_wave(const int* __restrict__ idx, const V* __restr
       v[u] = static_cast<Acc>(val[e + u]);
     }
 #pragma unroll
-    for (int u = 0; u < CH; ++u) g[u] = static_cast<Acc>(x[(static_cast<size_t>(c[u]) << 6) + b]);
+    for (int u = 0; u < CH; ++u) g[u] = static_cast<Acc>(vld<SPLIT, 6>(x, c[u], b));
 #pragma unroll
     for (int u = 0; u < CH; ++u) a[u & 3] += v[u] * g[u];
   }
@@ -141,7 +174,7 @@ __device__ inline Acc row_dot_wave(const int* __restrict__ idx, const V* __restr
       v[u] = e + u <= last ? w : static_cast<Acc>(0);
     }
 #pragma unroll
-    for (int u = 0; u < CH; ++u) g[u] = static_cast<Acc>(x[(static_cast<size_t>(c[u]) << 6) + b]);
+    for (int u = 0; u < CH; ++u) g[u] = static_cast<Acc>(vld<SPLIT, 6>(x, c[u], b));
 #pragma unroll
     for (int u = 0; u < CH; ++u) a[u & 3] += v[u] * g[u];
   }
@@ -150,11 +183,11 @@ __device__ inline Acc row_dot_wave(const int* __restrict__ idx, const V* __restr
 
 // Row dot product for the lane layout of LOGB (the B = 64 case uses the
 // wave-uniform path above; there the plan always has one lane per instance).
-template <int LOGB, typename Acc, typename V, typename X>
+template <int LOGB, typename Acc, bool SPLIT = false, typename V, typename X>
 __device__ inline Acc row_sum(const int* __restrict__ ptr, const int* __restrict__ idx, const V* __restrict__ val,
                               const X* __restrict__ x, const Slot& s, int b) {
   if constexpr (LOGB == 6) {
-    return row_dot_wave<Acc>(idx, val, x, ptr[s.row], ptr[s.row + 1], b);
+    return row_dot_wave<Acc, SPLIT>(idx, val, x, ptr[s.row], ptr[s.row + 1], b);
   } else {
     return row_dot<LOGB, Acc>(idx, val, x, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
   }
@@ -207,6 +240,13 @@ struct Ld {
   int ld, mask;
   __device__ size_t at(int i, int b) const { return static_cast<size_t>(i) * ld + (b & mask); }
 };
+
+// Case data of row i for lane b: one value shared by all lanes (mask 0) or
+// one per lane in the run's vector layout.
+template <bool SPLIT, int LOGB, typename T>
+__device__ inline T cld(const T* __restrict__ p, const Ld& l, int i, int b) {
+  return l.mask ? vld<SPLIT, LOGB>(p, i, b) : p[i];
+}
 
 
 // Check logic of one instance from its nine fixed-order sums S (column side:

@@ -52,6 +52,7 @@ __global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict
                                              const T* __restrict__ lo, const T* __restrict__ up, Ld lb, Inst st,
                                              int tloc, double rho, double* __restrict__ part, int nblk) {
   constexpr int B = 1 << LOGB;
+  constexpr bool SPL = kSplit<T, LOGB>;
   const int bid = block_id(plan);
   const Slot s = locate<LOGB>(plan, bid);
   const int b = threadIdx.x & (B - 1);
@@ -61,27 +62,28 @@ __global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict
   const bool rst = st.applyAt[b] == it;
   const T* __restrict__ ysrc = rst ? yp : y;
   T acc = 0;
-  if (s.row >= 0) acc = row_sum<LOGB, T>(ptr, idx, val, ysrc, s, b);
+  if (s.row >= 0) acc = row_sum<LOGB, T, SPL>(ptr, idx, val, ysrc, s, b);
   acc = reduce_sub<LOGB>(acc, s.logS);
   double q[2] = {0.0, 0.0};
   if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
     const int j = s.row;
-    const size_t t = (static_cast<size_t>(j) << LOGB) + b;
-    const T xin = rst ? xp[t] : x[t];
+    const T xin = vld<SPL, LOGB>(rst ? xp : x, j, b);
     T xo;
     if (rst) {
       xo = xin;
-      x0[t] = xin;
+      vst<SPL, LOGB>(x0, j, b, xin);
     } else {
-      xo = x0[t];
+      xo = vld<SPL, LOGB>(x0, j, b);
     }
     const T tau = static_cast<T>(st.tau[b]);
-    const T v = clamp_to<T>(xin - tau * (c[lc.at(j, b)] - acc), lo[lb.at(j, b)], up[lb.at(j, b)]);
+    const T v = clamp_to<T>(xin - tau * (cld<SPL, LOGB>(c, lc, j, b) - acc), cld<SPL, LOGB>(lo, lb, j, b),
+                            cld<SPL, LOGB>(up, lb, j, b));
     const double kk = static_cast<double>(it - st.restartAt[b]);
     const double lam = (kk + 1.0) / (kk + 2.0);
-    x[t] = static_cast<T>(lam) * (static_cast<T>(1.0 + rho) * v - static_cast<T>(rho) * xin) +
-           static_cast<T>(1.0 - lam) * xo;
-    xp[t] = v;
+    vst<SPL, LOGB>(x, j, b,
+                   static_cast<T>(lam) * (static_cast<T>(1.0 + rho) * v - static_cast<T>(rho) * xin) +
+                       static_cast<T>(1.0 - lam) * xo);
+    vst<SPL, LOGB>(xp, j, b, v);
     if (CHECK) {
       const double d = static_cast<double>(xin) - static_cast<double>(v);
       const double e = static_cast<double>(v) - static_cast<double>(xo);
@@ -108,34 +110,34 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
                                              const double* __restrict__ rfac, Inst st, int tloc, double rho,
                                              double* __restrict__ part, int nblk) {
   constexpr int B = 1 << LOGB;
+  constexpr bool SPL = kSplit<T, LOGB>;
   const int bid = block_id(plan);
   const Slot s = locate<LOGB>(plan, bid);
   const int b = threadIdx.x & (B - 1);
   T acc = 0;
   double acc64 = 0;
   if (s.row >= 0) {
-    acc = row_sum<LOGB, T>(ptr, idx, val, xp, s, b);
-    if (CHECK) acc64 = row_sum<LOGB, double>(ptr, idx, val64, xp, s, b);
+    acc = row_sum<LOGB, T, SPL>(ptr, idx, val, xp, s, b);
+    if (CHECK) acc64 = row_sum<LOGB, double, SPL>(ptr, idx, val64, xp, s, b);
   }
   acc = reduce_sub<LOGB>(acc, s.logS);
   if (CHECK) acc64 = reduce_sub<LOGB>(acc64, s.logS);
   double q[4] = {0.0, 0.0, 0.0, 0.0};
   if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
     const int i = s.row;
-    const size_t t = (static_cast<size_t>(i) << LOGB) + b;
     const long long it = st.iter[b] + tloc;
     const bool rst = st.applyAt[b] == it;
-    const T yin = rst ? yp[t] : y[t];
-    const T axin = rst ? axp[t] : ax[t];
+    const T yin = vld<SPL, LOGB>(rst ? yp : y, i, b);
+    const T axin = vld<SPL, LOGB>(rst ? axp : ax, i, b);
     T yo, axo;
     if (rst) {
       yo = yin;
       axo = axin;
-      y0[t] = yin;
-      ax0[t] = axin;
+      vst<SPL, LOGB>(y0, i, b, yin);
+      vst<SPL, LOGB>(ax0, i, b, axin);
     } else {
-      yo = y0[t];
-      axo = ax0[t];
+      yo = vld<SPL, LOGB>(y0, i, b);
+      axo = vld<SPL, LOGB>(ax0, i, b);
     }
     // Reflected dual step y+ = v + sigma clamp(-v/sigma, L, U), v = y - sigma g,
     // g = 2 ax+ - ax (reference pdhg.cpp:126-137), evaluated as
@@ -145,17 +147,17 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
     const T sg = static_cast<T>(st.sigma[b]);
     const T g = static_cast<T>(2) * acc - axin;
     const T tq = yin / sg;
-    const T dlo = g - rl[lr.at(i, b)], dhi = g - ru[lr.at(i, b)];
+    const T dlo = g - cld<SPL, LOGB>(rl, lr, i, b), dhi = g - cld<SPL, LOGB>(ru, lr, i, b);
     const T yplus = dlo < tq ? yin - sg * dlo : (dhi > tq ? yin - sg * dhi : static_cast<T>(0));
     const double kk = static_cast<double>(it - st.restartAt[b]);
     const double lam = (kk + 1.0) / (kk + 2.0);
     const T lamT = static_cast<T>(lam), omT = static_cast<T>(1.0 - lam);
     const T r1 = static_cast<T>(1.0 + rho), r0 = static_cast<T>(rho);
-    y[t] = lamT * (r1 * yplus - r0 * yin) + omT * yo;
-    ax[t] = lamT * (r1 * acc - r0 * axin) + omT * axo;
+    vst<SPL, LOGB>(y, i, b, lamT * (r1 * yplus - r0 * yin) + omT * yo);
+    vst<SPL, LOGB>(ax, i, b, lamT * (r1 * acc - r0 * axin) + omT * axo);
     if (CHECK) {
-      yp[t] = yplus;
-      axp[t] = acc;
+      vst<SPL, LOGB>(yp, i, b, yplus);
+      vst<SPL, LOGB>(axp, i, b, acc);
       const double L = rl64[lr.at(i, b)], U = ru64[lr.at(i, b)];
       const double res = (fmax(L - acc64, 0.0) + fmax(acc64 - U, 0.0)) * rfac[i];
       const double yy = static_cast<double>(yplus);
@@ -185,8 +187,9 @@ __global__ __launch_bounds__(kWG) void k_colcheck(RowPlan plan, const int* __res
   const int bid = block_id(plan);
   const Slot s = locate<LOGB>(plan, bid);
   const int b = threadIdx.x & (B - 1);
+  constexpr bool SPL = kSplit<T, LOGB>;
   double acc = 0;
-  if (s.row >= 0) acc = row_sum<LOGB, double>(ptr, idx, val64, yp, s, b);
+  if (s.row >= 0) acc = row_sum<LOGB, double, SPL>(ptr, idx, val64, yp, s, b);
   acc = reduce_sub<LOGB>(acc, s.logS);
   double q[3] = {0.0, 0.0, 0.0};
   if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
@@ -200,7 +203,7 @@ __global__ __launch_bounds__(kWG) void k_colcheck(RowPlan plan, const int* __res
     else if (!hu) res = fmax(-d, 0.0);
     res *= cfac[j];
     q[0] = res * res;
-    q[1] = cj * static_cast<double>(xp[(static_cast<size_t>(j) << LOGB) + b]);
+    q[1] = cj * static_cast<double>(vld<SPL, LOGB>(xp, j, b));
     if (d > 0 && hl) q[2] = d * l;
     else if (d < 0 && hu) q[2] = d * u;
   }
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(kWG) void k_finalize(Inst st, const double* __restr
 // ---------------------------------------------------------------------------
 // Op-level kernels (the Backend vtable, reference pdhg.cpp:51-70)
 // ---------------------------------------------------------------------------
-template <typename T, int LOGB>
+template <typename T, int LOGB, bool SPL>
 __global__ __launch_bounds__(kWG) void k_spmv(RowPlan plan, const int* __restrict__ ptr, const int* __restrict__ idx,
                                               const T* __restrict__ val, const T* __restrict__ in,
                                               T* __restrict__ out) {
@@ -261,9 +264,9 @@ __global__ __launch_bounds__(kWG) void k_spmv(RowPlan plan, const int* __restric
   const Slot s = locate<LOGB>(plan, bid);
   const int b = threadIdx.x & (B - 1);
   T acc = 0;
-  if (s.row >= 0) acc = row_sum<LOGB, T>(ptr, idx, val, in, s, b);
+  if (s.row >= 0) acc = row_sum<LOGB, T, SPL>(ptr, idx, val, in, s, b);
   acc = reduce_sub<LOGB>(acc, s.logS);
-  if (s.row >= 0 && s.sub == 0) out[(static_cast<size_t>(s.row) << LOGB) + b] = acc;
+  if (s.row >= 0 && s.sub == 0) vst<SPL, LOGB>(out, s.row, b, acc);
 }
 
 template <typename T>
@@ -520,7 +523,8 @@ struct Kern {
   ColFn col, colChk;
   RowFn row, rowChk;
   ColCheckFn colcheck;
-  SpmvFn spmv;
+  SpmvFn spmv;     // the run's vector layout
+  SpmvFn spmvStd;  // element (i, b) at i*B + b (op-level backend)
 };
 
 template <typename T, int LOGB>
@@ -531,7 +535,8 @@ Kern<T> kern_for() {
   k.row = k_row<T, LOGB, false>;
   k.rowChk = k_row<T, LOGB, true>;
   k.colcheck = k_colcheck<T, LOGB>;
-  k.spmv = k_spmv<T, LOGB>;
+  k.spmv = k_spmv<T, LOGB, kSplit<T, LOGB>>;
+  k.spmvStd = k_spmv<T, LOGB, false>;
   return k;
 }
 
@@ -562,6 +567,7 @@ class RunImpl final : public DeviceRun {
     B_ = s.B;
     logB_ = ilog2(B_);
     PDHG_REQUIRE((1 << logB_) == B_ && B_ <= 64, "batch width must be a power of two <= 64");
+    split_ = sizeof(T) == 8 && B_ == 64;
     k_ = kern_table<T>(logB_);
     nnz_ = P.A.nnz();
     // Matrices: FP32 (or FP64) values for the iterations, FP64 for the checks.
@@ -649,17 +655,17 @@ class RunImpl final : public DeviceRun {
   void upload(const double* c, const double* lo, const double* up, const double* rl, const double* ru, const double* x,
               const double* y) override {
     const size_t nB = static_cast<size_t>(n_) * B_, mB = static_cast<size_t>(m_) * B_;
-    put_both(c_, c64_, c, nc_);
-    put_both(lo_, lo64_, lo, nbnd_);
-    put_both(up_, up64_, up, nbnd_);
-    put_both(rl_, rl64_, rl, mr_);
-    put_both(ru_, ru64_, ru, mr_);
-    put(x_, x, nB);
-    put(x0_, x, nB);
-    put(xp_, x, nB);
-    put(y_, y, mB);
-    put(y0_, y, mB);
-    put(yp_, y, mB);
+    put_both(c_, c64_, c, nc_, lc_.mask != 0);
+    put_both(lo_, lo64_, lo, nbnd_, lb_.mask != 0);
+    put_both(up_, up64_, up, nbnd_, lb_.mask != 0);
+    put_both(rl_, rl64_, rl, mr_, lr_.mask != 0);
+    put_both(ru_, ru64_, ru, mr_, lr_.mask != 0);
+    put(x_, x, nB, true);
+    put(x0_, x, nB, true);
+    put(xp_, x, nB, true);
+    put(y_, y, mB, true);
+    put(y0_, y, mB, true);
+    put(yp_, y, mB, true);
     HIP_OK(hipStreamSynchronize(stream_));
   }
 
@@ -734,6 +740,7 @@ class RunImpl final : public DeviceRun {
     std::vector<T> tmp(len);
     HIP_OK(hipMemcpyAsync(tmp.data(), which == 0 ? xp_ : yp_, len * sizeof(T), hipMemcpyDeviceToHost, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
+    from_device_layout(tmp);
     for (size_t t = 0; t < len; ++t) out[t] = static_cast<double>(tmp[t]);
     bytesDown += len * sizeof(T);
   }
@@ -788,6 +795,7 @@ class RunImpl final : public DeviceRun {
       std::vector<T> full(static_cast<size_t>(rows) * B_);
       HIP_OK(hipMemcpyAsync(full.data(), d, full.size() * sizeof(T), hipMemcpyDeviceToHost, stream_));
       HIP_OK(hipStreamSynchronize(stream_));
+      from_device_layout(full);
       bytesDown += full.size() * sizeof(T);
       out.resize(static_cast<size_t>(rows) * k);
       for (int i = 0; i < rows; ++i)
@@ -830,9 +838,11 @@ class RunImpl final : public DeviceRun {
       std::vector<T> full(static_cast<size_t>(rows) * B_);
       HIP_OK(hipMemcpyAsync(full.data(), d, full.size() * sizeof(T), hipMemcpyDeviceToHost, stream_));
       HIP_OK(hipStreamSynchronize(stream_));
+      from_device_layout(full);
       for (int i = 0; i < rows; ++i)
         for (int q = 0; q < k; ++q)
           full[static_cast<size_t>(i) * B_ + q] = static_cast<T>(src[static_cast<size_t>(i) * k + q]);
+      to_device_layout(full);
       copy_h2d(d, full.data(), full.size());
     };
     place(x_, n_, in.x);
@@ -887,12 +897,41 @@ class RunImpl final : public DeviceRun {
     copy_h2d(d, t.data(), t.size());
     return d;
   }
-  void put(T* d, const double* h, size_t count) {
+  // Interleaved rows x B vectors between element order (i*B + b) and the
+  // device layout (the split layout for FP64 at B = 64, kSplit).
+  void to_device_layout(std::vector<T>& v) const {
+    if (!split_) return;
+    uint32_t w[128];
+    for (size_t r = 0; r + 64 <= v.size(); r += 64) {
+      for (int b = 0; b < 64; ++b) {
+        uint64_t bits;
+        std::memcpy(&bits, &v[r + b], 8);
+        w[b] = static_cast<uint32_t>(bits);
+        w[64 + b] = static_cast<uint32_t>(bits >> 32);
+      }
+      std::memcpy(&v[r], w, sizeof w);
+    }
+  }
+  void from_device_layout(std::vector<T>& v) const {
+    if (!split_) return;
+    uint32_t w[128];
+    for (size_t r = 0; r + 64 <= v.size(); r += 64) {
+      std::memcpy(w, &v[r], sizeof w);
+      for (int b = 0; b < 64; ++b) {
+        const uint64_t bits = (static_cast<uint64_t>(w[64 + b]) << 32) | w[b];
+        std::memcpy(&v[r + b], &bits, 8);
+      }
+    }
+  }
+  void put(T* d, const double* h, size_t count, bool interleaved) {
     std::vector<T> t(h, h + count);
+    if (interleaved) to_device_layout(t);
     copy_h2d(d, t.data(), count);
   }
-  void put_both(T* d, double* d64, const double* h, size_t count) {
-    put(d, h, count);
+  // Case data: the iteration copy in the run's layout, the FP64 copy (read by
+  // the check kernels) in element order.
+  void put_both(T* d, double* d64, const double* h, size_t count, bool interleaved) {
+    put(d, h, count, interleaved);
     copy_h2d(d64, h, count);
   }
 
@@ -943,6 +982,7 @@ class RunImpl final : public DeviceRun {
   Buffers buf_;
   Kern<T> k_{};
   int m_ = 0, n_ = 0, B_ = 1, logB_ = 0, nnz_ = 0;
+  bool split_ = false;
   int *aPtr_ = nullptr, *aIdx_ = nullptr, *tPtr_ = nullptr, *tIdx_ = nullptr;
   T *aVal_ = nullptr, *tVal_ = nullptr;
   double *aVal64_ = nullptr, *tVal64_ = nullptr, *rfac_ = nullptr, *cfac_ = nullptr;
@@ -1049,7 +1089,7 @@ class OpBackend final : public OpBackendBase {
     check_vec(in);
     check_vec(out);
     const PlanHost& p = transpose ? colPlan_ : rowPlan_;
-    hipLaunchKernelGGL(kern_.spmv, dim3(p.blocks), dim3(kWG), 0, stream_, p.plan, transpose ? tPtr_ : aPtr_,
+    hipLaunchKernelGGL(kern_.spmvStd, dim3(p.blocks), dim3(kWG), 0, stream_, p.plan, transpose ? tPtr_ : aPtr_,
                        transpose ? tIdx_ : aIdx_, transpose ? tVal_ : aVal_, vec_[in], vec_[out]);
     HIP_OK(hipGetLastError());
   }

@@ -131,6 +131,36 @@ def test_afiro_batch_equals_individual(gpu):
             assert abs(res[k].objective - ref[k]["objective"]) <= 1e-5 * (1 + abs(ref[k]["objective"]))
 
 
+_BATCH64 = {}
+
+
+def batch64_case():
+    if not _BATCH64:
+        lp = P.gen_random_lp(11, 120, 240, 4.0, 0.0)
+        rng = np.random.default_rng(3)
+        # Non-negative cost increments keep every case bounded (x >= 0).
+        costs = lp.c[None, :] + rng.uniform(0.0, 0.2, (64, lp.n))
+        _BATCH64["v"] = (lp, costs, O.oracle_solve(lp, costs, tol=1e-7))
+    return _BATCH64["v"]
+
+
+@pytest.mark.parametrize("compact", [0, 1])
+def test_batch64_fp64_matches_oracle(gpu, compact):
+    """A full-width FP64 run (64 lanes, one wavefront per row, split-word
+    vector layout) follows the oracle's batched solve case by case; with
+    compaction the live cases also move between runs of narrower widths."""
+    lp, costs, (ref, xs_o, ys_o, _) = batch64_case()
+    res, st, _ = P.solve_batch(lp, costs, P.default_options(tolerance=1e-7, precision=P.FP64, compact=compact))
+    assert st["batch"] == 64
+    for k in range(64):
+        assert ref[k]["status"] == 1 and res[k].status == P.OPTIMAL, k
+        assert abs(res[k].iterations - ref[k]["iterations"]) <= 4 * 64 + 0.02 * ref[k]["iterations"], k
+        obj = ref[k]["objective"]
+        assert abs(res[k].objective - obj) <= 1e-6 * (1 + abs(obj)), k
+        assert np.linalg.norm(res[k].y - ys_o[k]) <= 1e-4 * (1 + np.linalg.norm(ys_o[k])), k
+        assert np.linalg.norm(res[k].x - xs_o[k]) <= 1e-4 * (1 + np.linalg.norm(xs_o[k])), k
+
+
 def small_refinery():
     return P.gen_refinery(3, periods=2, streams=12, crudes=4, products=4, qualities=5, decades=3.0)
 
