@@ -52,23 +52,26 @@ def fit_warm_start(lp, n_train, tol, opts, steps):
     th, costs = P.gen_family(lp, SEED_TRAIN, n_train, 0.3)
     o = P.default_options(tolerance=tol, precision=opts.precision, max_iterations=opts.max_iterations,
                           progress_seconds=15.0)
-    xs, ys = [], []
+    xs, ys, ws_ = [], [], []
     for s in range(0, n_train, 64):
         res, st, _ = P.solve_batch(lp, costs[s:s + 64], o)
         xs += [r.x for r in res]
         ys += [r.y for r in res]
+        ws_ += [r.primal_weight for r in res]
         log(f"warm-start snapshots {s + len(res)}/{n_train}: {st['iterations']} iterations")
-    ws = W.WarmStart(lp, rank=32, hidden=128).fit(th, np.stack(xs), np.stack(ys), steps=steps, lr=1e-3)
+    ws = W.WarmStart(lp, rank=32, hidden=128).fit(th, np.stack(xs), np.stack(ys), steps=steps, lr=1e-3,
+                                                 omegas=np.array(ws_))
     log(f"warm-start MLP: loss {ws.losses[0]:.3g} -> {ws.losses[-1]:.3g}, POD energy kept "
         f"{ws.energy[:ws.used].sum() / max(ws.energy.sum(), 1e-300):.6f}")
     return ws
 
 
 def family_step(lp, thetas, costs, opts, ws=None, threads=16):
-    x0 = y0 = None
+    x0 = y0 = w0 = None
     if ws is not None:
         x0, y0 = ws.predict(thetas)
-    res, st, _ = P.solve_batch(lp, costs, opts, x0=x0, y0=y0)
+        w0 = ws.primal_weights(thetas)
+    res, st, _ = P.solve_batch(lp, costs, opts, x0=x0, y0=y0, primal_weight=w0)
     xs = np.stack([r.x for r in res])
     ys = np.stack([r.y for r in res])
     obj = np.array([r.objective for r in res])
@@ -108,6 +111,49 @@ def cpu_baseline(lp, costs, gpu_iters, seconds, threads):
     }
 
 
+def calibrate(threads, out_path):
+    """The reference's CPU/GPU crossover sweep (calibrateBandwidth,
+    src/parametric/parametric.cpp:537-590) on this machine: seconds per
+    iteration of the reference's own CPU PDHG (oracle/_ref, `threads` host
+    threads) and of this engine (FP64 and FP32 iterates) on the six sweep LPs,
+    then the router re-fit (router_fit.refit) for the engine's default
+    precision."""
+    import router_fit as RF
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    if not O.ref_available():
+        raise SystemExit("calibration needs oracle/_ref (the reference CPU PDHG built from its sources)")
+    O.ref_set_threads(threads)
+    rows = []
+    for s in RF.SIZES:
+        lp = RF.sweep_lp(s)
+        r = {"rows": lp.m, "cols": lp.n, "nnz": lp.nnz}
+        ref, _, _ = O.ref_solve(lp, tol=1e-30, max_iter=RF.ITERATIONS)
+        per = ref["solve_seconds"] / max(1, ref["iterations"])
+        r["cpu"] = {"seconds_per_iteration": per, "effective_GBs": RF.effective_gbs(lp, per),
+                    "setup_seconds": ref["setup_seconds"]}
+        r["gpu_fp64"] = RF.gpu_timing(lp, P.FP64)
+        r["gpu_fp32"] = RF.gpu_timing(lp, P.FP32)
+        rows.append(r)
+        log(f"calibrate s={s}: nnz {lp.nnz}, cpu {per * 1e6:.1f} us/it, gpu fp64 "
+            f"{r['gpu_fp64']['seconds_per_iteration'] * 1e6:.1f} us/it, fp32 "
+            f"{r['gpu_fp32']['seconds_per_iteration'] * 1e6:.1f} us/it")
+    out = {"gpu": P.gpu_info(), "cpu_threads": threads, "host_nproc": os.cpu_count(),
+           "iterations_per_point": RF.ITERATIONS, "pdhg_iteration_timing": rows,
+           "fp64": RF.refit(rows, "gpu_fp64"), "fp32": RF.refit(rows, "gpu_fp32")}
+    out["router_model"] = {"weights": out["fp32"]["weights"]}  # the engine's default precision
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    log(f"crossover nnz (measured): fp64 {out['fp64']['crossover_nnz_measured']}, "
+        f"fp32 {out['fp32']['crossover_nnz_measured']}; written to {out_path}")
+    print(json.dumps({"calibration": out_path, "fp64": out["fp64"]["ratio_fit"],
+                      "fp32": out["fp32"]["ratio_fit"],
+                      "crossover_nnz_measured": {"fp64": out["fp64"]["crossover_nnz_measured"],
+                                                 "fp32": out["fp32"]["crossover_nnz_measured"]}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,8 +161,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--precision", choices=["fp32", "fp64", "fp32auto"], default="fp64")
     ap.add_argument("--cold", action="store_true", help="no learned warm start (batched cold start only)")
-    ap.add_argument("--train", type=int, default=128, help="family parameters solved to fit the warm start")
-    ap.add_argument("--train-tol", type=float, default=1e-4, help="tolerance of the training solves")
+    ap.add_argument("--train", type=int, default=64, help="family parameters solved to fit the warm start")
+    ap.add_argument("--train-tol", type=float, default=1e-6, help="tolerance of the training solves")
     ap.add_argument("--train-steps", type=int, default=3000, help="Adam steps of the MLP")
     ap.add_argument("--no-cold-batch", action="store_true", help="skip the cold-start batched comparison run")
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -128,6 +174,7 @@ def main():
                     help="cap per instance (the reference default, src/pdhg/pdhg.h:34); unconverged = unsolved")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device copy of MIB MiB (units of the HBM traffic counters)")
+    ap.add_argument("--calibrate", metavar="JSON", help="run the CPU/GPU crossover sweep and router re-fit only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched family shards by "
@@ -136,6 +183,9 @@ def main():
     info = P.gpu_info()
     if not info["available"]:
         raise SystemExit(f"no usable gfx950 device: {info}")
+    if args.calibrate:
+        calibrate(args.cpu_threads, args.calibrate)
+        return
     if args.copy_calibration:
         # A plain device-to-device copy of a known size: its FETCH_SIZE /
         # WRITE_SIZE counter reading calibrates the units of the HBM-traffic

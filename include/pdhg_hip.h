@@ -70,7 +70,10 @@ typedef struct pdhg_lp {
 /* One case of a family sharing A: LpCase (src/pdhg/pdhg.h:68-70).  NULL
  * means "same as the base model".  x0 / y0 (original space) are a warm
  * start, which the reference does not have (it hard-wires x0 = clamp(0),
- * y0 = 0, src/pdhg/pdhg.cpp:509,519); NULL keeps that cold start. */
+ * y0 = 0, src/pdhg/pdhg.cpp:509,519); NULL keeps that cold start.
+ * primal_weight > 0 replaces the initial primal weight ||c_hat|| / ||b_hat||
+ * (src/pdhg/pdhg.cpp:513-529), e.g. the weight a solve of a neighbouring
+ * case ended with; 0 keeps the reference's. */
 typedef struct pdhg_case {
   const double* cost;
   const double* col_lower;
@@ -79,6 +82,7 @@ typedef struct pdhg_case {
   const double* row_upper;
   const double* x0;
   const double* y0;
+  double primal_weight;
 } pdhg_case;
 
 /* PdhgOptions (src/pdhg/pdhg.h:32-42) plus the engine's own fields. */
@@ -127,6 +131,15 @@ typedef struct pdhg_stats {
   int trace_length;
   char backend[32];
   char device_name[64];
+  /* Row mapping of the iteration kernels of the first run, per direction
+   * (column kernel over A^T, row kernel over A): nnz-per-lane target, 1 =
+   * row-length buckets / 2 = one mapping, XCD remap on/off.  autotuned = 1
+   * when the options left all three on auto and the autotuner chose them
+   * (timed on the device at the run's width, cached per matrix and width). */
+  int autotuned;
+  int col_nnz_per_lane, col_row_map, col_xcd;
+  int row_nnz_per_lane, row_row_map, row_xcd;
+  double tune_seconds;
 } pdhg_stats;
 
 /* PdhgResult (src/pdhg/pdhg.h:58-65).  x (n) and row_dual (m) are

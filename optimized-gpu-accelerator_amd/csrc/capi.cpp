@@ -4,6 +4,7 @@
 // (src/api/c_api.cpp:19,275-278).
 #include <algorithm>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -132,6 +133,9 @@ int pdhg_solve_batch(const pdhg_lp* base, const pdhg_case* cases, int k, const p
         cs[b].U = cases[b].row_upper;
         cs[b].x0 = cases[b].x0;
         cs[b].y0 = cases[b].y0;
+        PDHG_REQUIRE(std::isfinite(cases[b].primal_weight) && cases[b].primal_weight >= 0,
+                     "primal_weight must be finite and >= 0");
+        cs[b].omega0 = cases[b].primal_weight;
       }
     pdhgamd::solve_batch(md, cs, o, results, batch_stats);
   });

@@ -92,6 +92,13 @@ class DeviceRun {
   // Overwrites iterates and per-instance state of lanes 0..in.count-1 (after
   // upload() and init_state() of this run).
   virtual void import_lanes(const LaneState& in) = 0;
+  // Autotuner: times the column and the row iteration kernel under each
+  // candidate row mapping (`reps` launches each, HIP events on the run's
+  // stream) and keeps the fastest per direction.  Runs on scratch state, so
+  // it must come before upload() / init_state(); best[0] / best[1] get the
+  // chosen candidate index for the column / row kernel, ms[0] / ms[1] its
+  // mean launch time.
+  virtual void autotune(const Prepared& P, const std::vector<MapSetup>& cand, int reps, int* best, double* ms) = 0;
 
   long long launches = 0, graphLaunches = 0, bytesUp = 0, bytesDown = 0;
 };

@@ -5,6 +5,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "device.h"
 #include "prepare.h"
@@ -70,6 +73,65 @@ struct CaseScaled {  // per case: norms of the original vectors and initial omeg
   double cNorm = 0, rhsNorm = 0, omega = 1;
 };
 
+// ---- autotuner bookkeeping ------------------------------------------------
+struct TuneKey {
+  uint64_t shape;
+  int B, precision, device;
+  bool operator<(const TuneKey& r) const {
+    return std::tie(shape, B, precision, device) < std::tie(r.shape, r.B, r.precision, r.device);
+  }
+};
+
+std::mutex gTuneMutex;
+std::map<TuneKey, std::pair<MapSetup, MapSetup>> gTuned;
+
+bool tuned_maps(const TuneKey& k, MapSetup* col, MapSetup* row) {
+  std::lock_guard<std::mutex> lk(gTuneMutex);
+  auto it = gTuned.find(k);
+  if (it == gTuned.end()) return false;
+  *col = it->second.first;
+  *row = it->second.second;
+  return true;
+}
+
+void remember_maps(const TuneKey& k, const MapSetup& col, const MapSetup& row) {
+  std::lock_guard<std::mutex> lk(gTuneMutex);
+  gTuned[k] = {col, row};
+}
+
+// FNV-1a over the sparsity structure of the scaled A (shape, row pointers,
+// column indices): runs of the same matrix share their tuned mapping.
+uint64_t structure_hash(const Prepared& P) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) {
+    for (int i = 0; i < 8; ++i) {
+      h ^= (v >> (8 * i)) & 0xff;
+      h *= 1099511628211ull;
+    }
+  };
+  mix(static_cast<uint64_t>(P.m));
+  mix(static_cast<uint64_t>(P.n));
+  for (int v : P.A.ptr) mix(static_cast<uint32_t>(v));
+  for (int v : P.A.idx) mix(static_cast<uint32_t>(v));
+  return h;
+}
+
+// Row mappings the autotuner times.  At B = 64 a wavefront serves one row
+// (lane = instance), so only the XCD remap is open; below, the lanes per row
+// (nnz-per-lane target, which sets the row tile S*B) and bucketing are too.
+std::vector<MapSetup> map_candidates(int B) {
+  std::vector<MapSetup> c;
+  for (bool xcd : {false, true}) {
+    if (B == 64) {
+      c.push_back(MapSetup{4, 1, xcd});
+      continue;
+    }
+    for (int per : {2, 4, 8, 16})
+      for (int map : {1, 2}) c.push_back(MapSetup{per, map, xcd});
+  }
+  return c;
+}
+
 }  // namespace
 
 void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pdhg_options& o, pdhg_result* results,
@@ -119,6 +181,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     cs[q].cNorm = std::sqrt(cn0);
     cs[q].rhsNorm = std::sqrt(bn0);
     if (cn > 1e-20 && bn > 1e-20) cs[q].omega = std::sqrt(cn) / std::sqrt(bn);
+    if (cases[q].omega0 > 0) cs[q].omega = cases[q].omega0;
   }
 
   // Lane data for cases `ids` in a run of width B.
@@ -187,6 +250,14 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
   int runPrec = o.precision == PDHG_FP64 ? PDHG_FP64 : PDHG_FP32;
   std::vector<double> refKkt(k, kInf);
   std::vector<int> lastGain(k, 0);
+  // Row mapping of the iteration kernels: the options' explicit choice, or
+  // (all three on auto) the autotuner's, cached per matrix structure, width,
+  // precision and device for the life of the process.
+  const bool autoMap = o.lanes_per_row == 0 && o.row_map == 0 && o.xcd_remap < 0;
+  const uint64_t shape = autoMap ? structure_hash(P) : 0;
+  bool mapsRecorded = false, tunedFirst = false;
+  MapSetup firstCol, firstRow;
+  double tuneSeconds = 0;
   auto make_run = [&](const std::vector<int>& ids, int B) {
     RunSetup rs;
     rs.m = m;
@@ -204,7 +275,26 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       ms->rowMap = o.row_map == 2 ? 2 : 1;
       ms->xcd = o.xcd_remap == 1 || (o.xcd_remap < 0 && B >= 16);
     }
+    const TuneKey key{shape, B, rs.precision, o.device};
+    const bool tuneNow = autoMap && !tuned_maps(key, &rs.colMap, &rs.rowMap);
     auto run = DeviceRun::create(o.device, P, rs);
+    if (tuneNow) {
+      const auto t0 = Clock::now();
+      const std::vector<MapSetup> cand = map_candidates(B);
+      int best[2];
+      double ms[2];
+      run->autotune(P, cand, 8, best, ms);
+      rs.colMap = cand[best[0]];
+      rs.rowMap = cand[best[1]];
+      remember_maps(key, rs.colMap, rs.rowMap);
+      tuneSeconds += since_s(t0);
+    }
+    if (!mapsRecorded) {
+      mapsRecorded = true;
+      tunedFirst = autoMap;
+      firstCol = rs.colMap;
+      firstRow = rs.rowMap;
+    }
     LaneData d = lane_data(ids, B);
     run->upload(d.c.data(), d.lo.data(), d.up.data(), d.rl.data(), d.ru.data(), d.x.data(), d.y.data());
     run->init_state(sc, d.omega.data(), d.cNorm.data(), d.rhsNorm.data(), d.pad.data());
@@ -412,6 +502,14 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     acc.spectral_norm = P.normA;
     std::snprintf(acc.backend, sizeof(acc.backend), "hip-gfx950");
     std::snprintf(acc.device_name, sizeof(acc.device_name), "%s", di.name.c_str());
+    acc.autotuned = tunedFirst ? 1 : 0;
+    acc.col_nnz_per_lane = firstCol.nnzPerLane;
+    acc.col_row_map = firstCol.rowMap;
+    acc.col_xcd = firstCol.xcd ? 1 : 0;
+    acc.row_nnz_per_lane = firstRow.nnzPerLane;
+    acc.row_row_map = firstRow.rowMap;
+    acc.row_xcd = firstRow.xcd ? 1 : 0;
+    acc.tune_seconds = tuneSeconds;
     *stats = acc;
   }
 }

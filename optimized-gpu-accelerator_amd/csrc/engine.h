@@ -14,6 +14,7 @@ namespace pdhgamd {
 struct CaseData {
   const double *c = nullptr, *l = nullptr, *u = nullptr, *L = nullptr, *U = nullptr;
   const double *x0 = nullptr, *y0 = nullptr;  // warm start (original space)
+  double omega0 = 0;                            // > 0: initial primal weight
 };
 
 // Runs all cases (any count; chunks of at most 64 share one set-up) and fills

@@ -831,6 +831,65 @@ class RunImpl final : public DeviceRun {
     sel(st_.restarts, o.restarts);
   }
 
+  void autotune(const Prepared& P, const std::vector<MapSetup>& cand, int reps, int* best, double* ms) override {
+    PDHG_REQUIRE(!cand.empty() && reps >= 1, "autotune: no candidates");
+    // Scratch state: zero vectors, every lane live, no restart pending.
+    const size_t nB = static_cast<size_t>(n_) * B_, mB = static_cast<size_t>(m_) * B_;
+    for (T* v : {x_, xp_, x0_}) HIP_OK(hipMemsetAsync(v, 0, nB * sizeof(T), stream_));
+    for (T* v : {y_, yp_, y0_, ax_, axp_, ax0_}) HIP_OK(hipMemsetAsync(v, 0, mB * sizeof(T), stream_));
+    HIP_OK(hipMemsetAsync(c_, 0, nc_ * sizeof(T), stream_));
+    for (T* v : {lo_, up_}) HIP_OK(hipMemsetAsync(v, 0, nbnd_ * sizeof(T), stream_));
+    for (T* v : {rl_, ru_}) HIP_OK(hipMemsetAsync(v, 0, mr_ * sizeof(T), stream_));
+    std::vector<int> zeros(B_, 0);
+    std::vector<long long> zl(B_, 0), none(B_, -1);
+    std::vector<double> step(B_, 1e-3);
+    copy_h2d(st_.done, zeros.data(), B_);
+    copy_h2d(st_.applyAt, none.data(), B_);
+    copy_h2d(st_.iter, zl.data(), B_);
+    copy_h2d(st_.restartAt, zl.data(), B_);
+    copy_h2d(st_.tau, step.data(), B_);
+    copy_h2d(st_.sigma, step.data(), B_);
+    int* scratch = buf_.alloc<int>(static_cast<size_t>(std::max(m_, n_)));
+    for (int dir = 0; dir < 2; ++dir) {
+      PlanHost& cur = dir == 0 ? colPlan_ : rowPlan_;
+      const Csr& M = dir == 0 ? P.AT : P.A;
+      double bestMs = 0;
+      int bestI = -1;
+      for (size_t i = 0; i < cand.size(); ++i) {
+        PlanHost h = build_plan(M, logB_, cand[i]);
+        if (!h.perm.empty()) copy_h2d(scratch, h.perm.data(), h.perm.size());
+        h.plan.perm = h.perm.empty() ? nullptr : scratch;
+        const PlanHost keep = cur;
+        cur = h;
+        auto one = [&] { dir == 0 ? launch_col(false, 0) : launch_row(false, 0); };
+        one();
+        HIP_OK(hipEventRecord(ev0_, stream_));
+        for (int r = 0; r < reps; ++r) one();
+        HIP_OK(hipEventRecord(ev1_, stream_));
+        HIP_OK(hipEventSynchronize(ev1_));
+        HIP_OK(hipGetLastError());
+        float el = 0;
+        HIP_OK(hipEventElapsedTime(&el, ev0_, ev1_));
+        cur = keep;
+        const double t = el / reps;
+        if (bestI < 0 || t < bestMs) {
+          bestMs = t;
+          bestI = static_cast<int>(i);
+        }
+      }
+      // Install the winner with its own permutation and partial buffers.
+      PlanHost h = build_plan(M, logB_, cand[bestI]);
+      if (!h.perm.empty()) h.dPerm = upload_int(h.perm);
+      h.plan.perm = h.dPerm;
+      cur = h;
+      best[dir] = bestI;
+      ms[dir] = bestMs;
+    }
+    colPart_ = buf_.alloc<double>(static_cast<size_t>(5) * B_ * colPlan_.blocks);
+    rowPart_ = buf_.alloc<double>(static_cast<size_t>(4) * B_ * rowPlan_.blocks);
+    launches = 0;
+  }
+
   void import_lanes(const LaneState& in) override {
     const int k = in.count;
     PDHG_REQUIRE(k <= B_, "import: more lanes than the run holds");

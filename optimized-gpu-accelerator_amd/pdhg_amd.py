@@ -36,7 +36,7 @@ class LpStruct(C.Structure):
 
 class CaseStruct(C.Structure):
     _fields_ = [("cost", _dp), ("col_lower", _dp), ("col_upper", _dp), ("row_lower", _dp), ("row_upper", _dp),
-                ("x0", _dp), ("y0", _dp)]
+                ("x0", _dp), ("y0", _dp), ("primal_weight", C.c_double)]
 
 
 class Options(C.Structure):
@@ -60,7 +60,10 @@ class Stats(C.Structure):
                 ("col_kernel_ms", C.c_double), ("row_kernel_ms", C.c_double), ("col_kernel_bytes", C.c_double),
                 ("row_kernel_bytes", C.c_double),
                 ("trace", _dp), ("trace_capacity", C.c_int), ("trace_length", C.c_int),
-                ("backend", C.c_char * 32), ("device_name", C.c_char * 64)]
+                ("backend", C.c_char * 32), ("device_name", C.c_char * 64),
+                ("autotuned", C.c_int), ("col_nnz_per_lane", C.c_int), ("col_row_map", C.c_int), ("col_xcd", C.c_int),
+                ("row_nnz_per_lane", C.c_int), ("row_row_map", C.c_int), ("row_xcd", C.c_int),
+                ("tune_seconds", C.c_double)]
 
 
 class ResultStruct(C.Structure):
@@ -323,8 +326,10 @@ def solve(lp: LP, opts: Options | None = None, trace_capacity: int = 0):
 
 
 def solve_batch(lp: LP, costs: np.ndarray | None = None, opts: Options | None = None, k: int | None = None,
-                x0: np.ndarray | None = None, y0: np.ndarray | None = None, trace_capacity: int = 0):
-    """solvePdhgBatch over cost cases (k x n) with optional warm starts (k x n, k x m)."""
+                x0: np.ndarray | None = None, y0: np.ndarray | None = None, trace_capacity: int = 0,
+                primal_weight=None):
+    """solvePdhgBatch over cost cases (k x n) with optional warm starts (k x n, k x m) and
+    initial primal weights (scalar or k values; None keeps the reference's)."""
     opts = opts or default_options()
     if costs is not None:
         costs = np.ascontiguousarray(costs, dtype=np.float64)
@@ -343,6 +348,8 @@ def solve_batch(lp: LP, costs: np.ndarray | None = None, opts: Options | None = 
             cases[b].x0 = _d(x0[b])
         if y0 is not None:
             cases[b].y0 = _d(y0[b])
+        if primal_weight is not None:
+            cases[b].primal_weight = float(np.broadcast_to(np.asarray(primal_weight, dtype=np.float64), (k,))[b])
     res = (ResultStruct * k)()
     for b in range(k):
         res[b].x, res[b].row_dual = _d(xs[b]), _d(ys[b])

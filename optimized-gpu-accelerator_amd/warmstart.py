@@ -11,6 +11,13 @@ solve as its starting point.  Every warm-started answer still has to reach the
 same KKT tolerance and pass the same certifier as a cold one; a cold start
 stays available (x0 = y0 = None).
 
+Primal weight: restarted PDHG adapts the primal weight omega along its run
+(src/pdhg/pdhg.cpp:640-650).  A start near the optimum with the cold-start
+weight ||c|| / ||b|| gains little, because near the optimum the weight updates
+see only small, noisy movements.  When the training solves' final weights are
+given, a second small MLP (same kernels) maps theta to log omega, and the
+prediction is passed as the cases' initial primal weight.
+
 The reference solver has no warm start: it hard-wires x0 = clamp(0) and
 y0 = 0 (src/pdhg/pdhg.cpp:509,519).
 """
@@ -36,8 +43,10 @@ class WarmStart:
     def _to_scaled(self, xs, ys):
         return np.hstack([xs / self.xs, ys / self.ys])
 
-    def fit(self, thetas: np.ndarray, xs: np.ndarray, ys: np.ndarray, steps: int = 3000, lr: float = 1e-3):
-        """POD of the solutions (xs k x n, ys k x m) and GPU training of theta -> coefficients."""
+    def fit(self, thetas: np.ndarray, xs: np.ndarray, ys: np.ndarray, steps: int = 3000, lr: float = 1e-3,
+            omegas: np.ndarray | None = None):
+        """POD of the solutions (xs k x n, ys k x m) and GPU training of theta -> coefficients;
+        with omegas (the training solves' final primal weights) also theta -> log omega."""
         Z = self._to_scaled(xs, ys)
         self.mean, self.basis, A, self.energy, used = P.pod_basis(Z, self.rank)
         self.used = used
@@ -46,7 +55,21 @@ class WarmStart:
         X = np.asarray(thetas, dtype=np.float64) / self.theta_scale
         self.mlp = P.Mlp(X.shape[1], self.hidden, self.rank, seed=self.seed)
         self.losses = self.mlp.train(X, T, steps=steps, lr=lr)
+        self.wnet = None
+        if omegas is not None:
+            lw = np.log(np.asarray(omegas, dtype=np.float64))
+            self.lw_mean, self.lw_scale = float(lw.mean()), float(max(lw.std(), 1e-12))
+            self.wnet = P.Mlp(X.shape[1], 64, 1, seed=self.seed + 1)
+            self.weight_losses = self.wnet.train(X, ((lw - self.lw_mean) / self.lw_scale)[:, None], steps=2000,
+                                                 lr=3e-3)
         return self
+
+    def primal_weights(self, thetas: np.ndarray) -> np.ndarray | None:
+        """Predicted initial primal weights (None when fitted without omegas)."""
+        if self.wnet is None:
+            return None
+        X = np.asarray(thetas, dtype=np.float64) / self.theta_scale
+        return np.exp(self.wnet.forward(X)[:, 0] * self.lw_scale + self.lw_mean)
 
     def coefficients(self, thetas: np.ndarray) -> np.ndarray:
         X = np.asarray(thetas, dtype=np.float64) / self.theta_scale

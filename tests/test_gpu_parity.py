@@ -225,3 +225,34 @@ def test_packed_uneven_batch_matches_oracle(gpu, prec):
         assert abs(r.objective - o[0]["objective"]) <= 1e-5 * (1 + abs(obj_star))
         if prec == P.FP64:
             assert abs(r.iterations - o[0]["iterations"]) <= 4 * 64 + 0.02 * o[0]["iterations"]
+
+
+def test_autotuner_choice_is_cached_and_keeps_results(gpu):
+    """With the row mapping on auto the first run times the candidate mappings
+    on the device and keeps the fastest per kernel; a second solve of the same
+    matrix reuses the choice; the answers equal those of a fixed mapping."""
+    lp = P.gen_random_lp(5, 3000, 6000, 5.12, 1.0)  # log-normal row lengths, ~30k iterations
+    costs = np.tile(lp.c, (8, 1))
+    auto = P.default_options(tolerance=1e-6, precision=P.FP64, kernel_timing_reps=50)
+    r1, s1, _ = P.solve_batch(lp, costs, auto)
+    r2, s2, _ = P.solve_batch(lp, costs, auto)
+    fixed = P.default_options(tolerance=1e-6, precision=P.FP64, kernel_timing_reps=50, lanes_per_row=4, row_map=1,
+                              xcd_remap=0)
+    r3, s3, _ = P.solve_batch(lp, costs, fixed)
+    assert s1["autotuned"] == 1 and s1["tune_seconds"] > 0
+    assert s2["autotuned"] == 1 and s2["tune_seconds"] == 0  # cached
+    assert s3["autotuned"] == 0 and s3["col_nnz_per_lane"] == 4 and s3["col_xcd"] == 0
+    for s in (s1, s2):
+        for d in ("col", "row"):
+            assert s[f"{d}_nnz_per_lane"] in (2, 4, 8, 16) and s[f"{d}_row_map"] in (1, 2)
+    assert (s1["col_nnz_per_lane"], s1["row_nnz_per_lane"]) == (s2["col_nnz_per_lane"], s2["row_nnz_per_lane"])
+    print(f"tuned col {s1['col_nnz_per_lane']}/{s1['col_row_map']}/{s1['col_xcd']} "
+          f"row {s1['row_nnz_per_lane']}/{s1['row_row_map']}/{s1['row_xcd']} in {s1['tune_seconds'] * 1e3:.1f} ms; "
+          f"kernels tuned {s1['col_kernel_ms'] * 1e3:.1f}+{s1['row_kernel_ms'] * 1e3:.1f} us, fixed "
+          f"{s3['col_kernel_ms'] * 1e3:.1f}+{s3['row_kernel_ms'] * 1e3:.1f} us")
+    # the fixed mapping is one of the candidates: the tuned one is not slower (timing noise allowed)
+    assert s1["col_kernel_ms"] + s1["row_kernel_ms"] <= 1.15 * (s3["col_kernel_ms"] + s3["row_kernel_ms"])
+    for a, b in zip(r1, r3):
+        assert a.status == b.status == P.OPTIMAL
+        assert abs(a.iterations - b.iterations) <= 4 * 64 + 0.02 * b.iterations
+        assert abs(a.objective - b.objective) <= 1e-6 * (1 + abs(b.objective))

@@ -75,18 +75,25 @@ def test_warm_start_beats_cold_start_on_held_out_family(gpu):
     train, _, _ = P.solve_batch(lp, c_train, opts)
     assert all(r.status == P.OPTIMAL for r in train)
     ws = W.WarmStart(lp, rank=32, hidden=128).fit(th_train, np.stack([r.x for r in train]),
-                                                 np.stack([r.y for r in train]), steps=2000, lr=1e-3)
+                                                 np.stack([r.y for r in train]), steps=2000, lr=1e-3,
+                                                 omegas=np.array([r.primal_weight for r in train]))
     assert ws.losses[-1] < 0.2 * ws.losses[0]
+    assert ws.weight_losses[-1] < ws.weight_losses[0]
     cold, _, _ = P.solve_batch(lp, c_eval, opts)
     x0, y0 = ws.predict(th_eval)
     warm, _, _ = P.solve_batch(lp, c_eval, opts, x0=x0, y0=y0)
     xm, ym = ws.predict(th_eval, coeffs=np.zeros((16, ws.rank)))
     mean_only, _, _ = P.solve_batch(lp, c_eval, opts, x0=xm, y0=ym)
+    w0 = ws.primal_weights(th_eval)
+    assert w0.shape == (16,) and np.all(np.isfinite(w0)) and np.all(w0 > 0)
+    warm_w, _, _ = P.solve_batch(lp, c_eval, opts, x0=x0, y0=y0, primal_weight=w0)
     ic = np.array([r.iterations for r in cold], float)
     iw = np.array([r.iterations for r in warm], float)
     im = np.array([r.iterations for r in mean_only], float)
-    print(f"iterations cold {ic.mean():.0f}  mean-of-snapshots start {im.mean():.0f}  learned start {iw.mean():.0f}")
-    for group in (cold, warm):
+    iww = np.array([r.iterations for r in warm_w], float)
+    print(f"iterations cold {ic.mean():.0f}  mean-of-snapshots start {im.mean():.0f}  learned start {iw.mean():.0f}"
+          f"  learned start + learned primal weight {iww.mean():.0f}")
+    for group in (cold, warm, warm_w):
         assert all(r.status == P.OPTIMAL and r.relative_kkt <= 1e-6 for r in group)
         cert = P.certify_batch(lp, c_eval, np.stack([r.x for r in group]), np.stack([r.y for r in group]),
                                np.array([r.objective for r in group]))
