@@ -373,8 +373,10 @@ class PackedRun {
     HIP_OK(hipEventCreate(&e1_));
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, device));
-    // Four workgroups per CU, a multiple of 8 so every queue has workgroups.
-    grid_ = std::max(8, (std::max(1, prop.multiProcessorCount) * 4 + 7) / 8 * 8);
+    // At most four workgroups per CU, a multiple of 8 so every queue has
+    // workgroups; set_tasks() shrinks the grid to the live task count.
+    maxGrid_ = std::max(8, (std::max(1, prop.multiProcessorCount) * 4 + 7) / 8 * 8);
+    grid_ = maxGrid_;
     std::vector<int> cperm, rperm;
     build_tasks(AT.ptr, colOff, nnzPerLane, colAll_, cperm, colBegAll_);
     build_tasks(A.ptr, rowOff, nnzPerLane, rowAll_, rperm, rowBegAll_);
@@ -572,6 +574,9 @@ class PackedRun {
     live_tasks(rowAll_, rowBegAll_, done, rt, rb, rowQ_);
     nCol_ = static_cast<int>(ct.size());
     nRow_ = static_cast<int>(rt.size());
+    // No more workgroups than tasks: a launch over a few live instances
+    // then starts a few workgroups instead of the full persistent grid.
+    grid_ = std::min(maxGrid_, std::max(8, (std::max(nCol_, nRow_) + 7) / 8 * 8));
     if (!ct.empty()) putv(colTasks_, ct);
     if (!rt.empty()) putv(rowTasks_, rt);
     putv(dColBeg_, cb);
@@ -621,7 +626,7 @@ class PackedRun {
   int M_, N_, K_, checkEvery_;
   double rho_;
   bool useGraph_;
-  int grid_ = 1, nCol_ = 0, nRow_ = 0;
+  int grid_ = 8, maxGrid_ = 8, nCol_ = 0, nRow_ = 0;
   double tol_ = 1e-6;
   hipStream_t s_ = nullptr;
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
