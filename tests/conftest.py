@@ -23,6 +23,15 @@ def load_netlib(name):
     return lp, float(d["optimum"])
 
 
+def load_plan(name):
+    """Planning-model fixture (tests/golden/<name>.npz, minimisation form) and
+    the reference's published optimum in that form."""
+    d = np.load(os.path.join(ROOT, "tests", "golden", f"{name}.npz"))
+    lp = pdhg_amd.LP(int(d["m"]), int(d["n"]), d["start"], d["index"], d["value"], d["c"], d["l"], d["u"], d["L"],
+                     d["U"], float(d["offset"]))
+    return lp, float(d["optimum"])
+
+
 @pytest.fixture(scope="session")
 def gpu():
     """The HIP device must be there when GPU tests run: no silent fallback."""

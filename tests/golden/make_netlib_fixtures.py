@@ -25,8 +25,10 @@ INF = float("inf")
 
 
 def read_mps(path):
-    """Minimal fixed/free MPS reader: ROWS, COLUMNS (MARKER lines skipped),
-    RHS (objective RHS = -offset), RANGES, BOUNDS (UP LO FX FR MI PL BV)."""
+    """Minimal fixed/free MPS reader: OBJSENSE, ROWS, COLUMNS (MARKER lines
+    skipped), RHS (objective RHS = -offset), RANGES, BOUNDS (UP LO FX FR MI PL
+    BV).  A maximisation model is returned in minimisation form (cost and
+    offset negated) with sense = -1."""
     opener = gzip.open if path.endswith(".gz") else open
     rows, rtype, obj = [], {}, None
     cols, colidx = [], {}
@@ -35,6 +37,7 @@ def read_mps(path):
     rhs, ranges = {}, {}
     lo, up = {}, {}
     offset = 0.0
+    sense = 1
     sec = None
     with opener(path, "rt") as f:
         for raw in f:
@@ -45,7 +48,9 @@ def read_mps(path):
                 sec = line.split()[0]
                 continue
             t = line.split()
-            if sec == "ROWS":
+            if sec == "OBJSENSE":
+                sense = -1 if t[0].upper().startswith("MAX") else 1
+            elif sec == "ROWS":
                 typ, name = t[0], t[1]
                 if typ == "N":
                     if obj is None:
@@ -125,7 +130,8 @@ def read_mps(path):
                 L[i] = b - abs(R)
             else:
                 U[i] = b + abs(R)
-    c = np.array([cost.get(j, 0.0) for j in range(n)])
+    c = sense * np.array([cost.get(j, 0.0) for j in range(n)])
+    offset = sense * offset
     l = np.array([lo.get(j, 0.0) for j in range(n)])
     u = np.array([up.get(j, INF) for j in range(n)])
     trip = sorted(((j, ridx[r], v) for (r, j), v in entries.items() if v != 0.0))
@@ -135,7 +141,8 @@ def read_mps(path):
     for j, _, _ in trip:
         start[j + 1] += 1
     start = np.cumsum(start).astype(np.int32)
-    return dict(m=m, n=n, start=start, index=index, value=value, c=c, l=l, u=u, L=L, U=U, offset=offset)
+    return dict(m=m, n=n, start=start, index=index, value=value, c=c, l=l, u=u, L=L, U=U, offset=offset,
+                sense=sense)
 
 
 def main():
@@ -143,6 +150,7 @@ def main():
     out = os.path.dirname(os.path.abspath(__file__))
     for name, opt in KNOWN.items():
         d = read_mps(os.path.join(ref, "data", "netlib", name + ".mps.gz"))
+        assert d.pop("sense") == 1
         np.savez(os.path.join(out, f"netlib_{name}.npz"), optimum=opt, **d)
         print(name, d["m"], d["n"], int(d["start"][-1]))
 

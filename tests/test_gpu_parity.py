@@ -256,3 +256,25 @@ def test_autotuner_choice_is_cached_and_keeps_results(gpu):
         assert a.status == b.status == P.OPTIMAL
         assert abs(a.iterations - b.iterations) <= 4 * 64 + 0.02 * b.iterations
         assert abs(a.objective - b.objective) <= 1e-6 * (1 + abs(b.objective))
+
+
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32_AUTO])
+@pytest.mark.parametrize("name", ["plan_6x4", "plan_10x12", "plan_20x12"])
+def test_plan_models_reach_published_optima(gpu, name, prec):
+    """The reference's own refinery planning models (data/gen of the reference)
+    reach the optimum it published for them, with a certificate."""
+    from conftest import load_plan
+
+    lp, opt = load_plan(name)
+    r, st, _ = P.solve(lp, P.default_options(tolerance=1e-6, precision=prec))
+    print(f"{name} prec={prec}: {r.iterations} iterations, objective {r.objective:.10g} vs published {opt:.10g}, "
+          f"switch {st['fp64_switch_iteration']}")
+    assert r.status == P.OPTIMAL and r.relative_kkt <= 1e-6
+    assert abs(r.objective - opt) <= 1e-5 * (1 + abs(opt))
+    lb, _, _ = P.safe_dual_bound(lp, r.y)
+    assert lb <= opt + 1e-9 * (1 + abs(opt))
+    cert = P.certify(lp, r.x, r.y, r.objective)
+    assert cert["accepted"], cert
+    if prec == P.FP64 and name == "plan_6x4":
+        o, _, _, _ = O.oracle_solve(lp, tol=1e-6)
+        assert abs(r.iterations - o[0]["iterations"]) <= 4 * 64 + 0.02 * o[0]["iterations"]

@@ -13,7 +13,7 @@ import pytest
 
 import oracle_lib as O
 import pdhg_amd as P
-from conftest import load_netlib
+from conftest import load_netlib, load_plan
 
 needs_ref = pytest.mark.skipif(not O.ref_available(), reason="compiled reference (oracle/_ref) not built")
 
@@ -133,6 +133,41 @@ def test_safe_bound_bitwise_equals_reference():
         y = lp.ystar + 1e-3 * rng.standard_normal(lp.m)
         assert O.oracle_safe_bound(lp2, y) == O.ref_safe_bound(lp2, y)
     assert O.oracle_safe_bound(lp, lp.ystar) == O.ref_safe_bound(lp, lp.ystar)
+
+
+def highs_objective(lp):
+    from scipy.optimize import linprog
+    import scipy.sparse as sps
+
+    A = lp.scipy_csc()
+    fin = lambda v: np.isfinite(v) & (np.abs(v) < 1e20)  # noqa: E731
+    eq = fin(lp.L) & fin(lp.U) & (lp.L == lp.U)
+    ub_rows = fin(lp.U) & ~eq
+    lb_rows = fin(lp.L) & ~eq
+    A_ub = sps.vstack([A[ub_rows], -A[lb_rows]])
+    b_ub = np.concatenate([lp.U[ub_rows], -lp.L[lb_rows]])
+    bounds = [(lo if fin(np.array(lo)) else None, hi if fin(np.array(hi)) else None) for lo, hi in zip(lp.l, lp.u)]
+    r = linprog(lp.c, A_ub=A_ub, b_ub=b_ub, A_eq=A[eq], b_eq=lp.L[eq], bounds=bounds, method="highs")
+    assert r.status == 0
+    return r.fun + lp.offset
+
+
+@pytest.mark.parametrize("name", ["plan_6x4", "plan_10x12", "plan_20x12"])
+def test_plan_fixtures_match_published_optima(name):
+    """The parsed reference planning models reproduce the optimum the reference
+    published for them (results/gpu/planning_engines.csv, dual simplex)."""
+    lp, opt = load_plan(name)
+    assert abs(highs_objective(lp) - opt) <= 1e-7 * (1 + abs(opt))
+
+
+def test_oracle_reaches_published_plan_optimum():
+    """The oracle PDHG on the reference's plan_6x4 model at rel-KKT 1e-6."""
+    lp, opt = load_plan("plan_6x4")
+    r, _, ys, _ = O.oracle_solve(lp, tol=1e-6)
+    assert r[0]["status"] == 1
+    assert abs(r[0]["objective"] - opt) <= 1e-5 * (1 + abs(opt))
+    lb, _, _ = P.safe_dual_bound(lp, ys[0])
+    assert lb <= opt + 1e-9 * (1 + abs(opt))
 
 
 def test_netlib_fixtures_match_highs():
