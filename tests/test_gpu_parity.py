@@ -278,3 +278,46 @@ def test_plan_models_reach_published_optima(gpu, name, prec):
     if prec == P.FP64 and name == "plan_6x4":
         o, _, _, _ = O.oracle_solve(lp, tol=1e-6)
         assert abs(r.iterations - o[0]["iterations"]) <= 4 * 64 + 0.02 * o[0]["iterations"]
+
+
+@pytest.mark.parametrize("prec,tol", [(P.FP64, 1e-4), (P.FP32, 1e-4), (P.FP64, 1e-6), (P.FP32_AUTO, 1e-6)])
+def test_lognormal_lp_full_size(gpu, prec, tol):
+    """Single LP at full size (m=50,000, n=100,000, nnz ~5.12e5, log-normal row
+    lengths, sigma=1): size-independent checks instead of the oracle: the
+    generator's closed-form optimum and an independent FP64 host re-evaluation
+    of the KKT residuals (its columns have no upper bounds, so the safe dual
+    bound without implied bounds is not finite and pins nothing here)."""
+    lp = P.gen_random_lp(21, 50_000, 100_000, 5.12, 1.0)
+    obj_star = float(lp.c @ lp.xstar)
+    r, st, _ = P.solve(lp, P.default_options(tolerance=tol, precision=prec))
+    print(f"prec={prec} tol={tol}: {r.iterations} iterations, {st['solve_seconds']:.2f} s, objective error "
+          f"{abs(r.objective - obj_star) / (1 + abs(obj_star)):.2e}")
+    assert r.status == P.OPTIMAL and r.relative_kkt <= tol
+    relP, relD, gap, _ = host_kkt(lp, r.x, r.y)
+    assert max(relP, relD, gap) <= 1.5 * tol, (relP, relD, gap)
+    assert abs(r.objective - obj_star) <= 10 * tol * (1 + abs(obj_star))
+
+
+def test_refinery_lp_full_size(gpu):
+    """The full refinery LP (555,852 nonzeros, rows/columns scaled by
+    10^U(-3,3)), single instance: at rel-KKT 1e-4 with FP32 iterates (host
+    re-evaluation of the residuals), and at 1e-6 with FP64 and with FP32_AUTO
+    iterates, both accepted by the certifier and agreeing to its level."""
+    lp = P.gen_refinery(3)
+    r, st, _ = P.solve(lp, P.default_options(tolerance=1e-4, precision=P.FP32))
+    print(f"refinery 1e-4 FP32: {r.iterations} iterations, {st['solve_seconds']:.2f} s")
+    assert r.status == P.OPTIMAL and r.relative_kkt <= 1e-4
+    relP, relD, gap, _ = host_kkt(lp, r.x, r.y)
+    assert max(relP, relD, gap) <= 1.5e-4, (relP, relD, gap)
+    out = {}
+    for prec in (P.FP64, P.FP32_AUTO):
+        r, st, _ = P.solve(lp, P.default_options(tolerance=1e-6, precision=prec))
+        print(f"refinery 1e-6 prec={prec}: {r.iterations} iterations, {st['solve_seconds']:.2f} s, "
+              f"objective {r.objective:.12g}, FP64 from {st['fp64_switch_iteration']}")
+        assert r.status == P.OPTIMAL and r.relative_kkt <= 1e-6
+        relP, relD, gap, _ = host_kkt(lp, r.x, r.y)
+        assert max(relP, relD, gap) <= 1.5e-6, (prec, relP, relD, gap)
+        cert = P.certify(lp, r.x, r.y, r.objective)
+        assert cert["accepted"], cert
+        out[prec] = r.objective
+    assert abs(out[P.FP32_AUTO] - out[P.FP64]) <= 1e-4 * (1 + abs(out[P.FP64]))
