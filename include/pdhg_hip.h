@@ -132,8 +132,10 @@ typedef struct pdhg_stats {
   char backend[32];
   char device_name[64];
   /* Row mapping of the iteration kernels of the first run, per direction
-   * (column kernel over A^T, row kernel over A): nnz-per-lane target, 1 =
-   * row-length buckets / 2 = one mapping, XCD remap on/off.  autotuned = 1
+   * (column kernel over A^T, row kernel over A): nnz-per-lane target (0 =
+   * one row per B-lane group, rows' entries read through the scalar cache),
+   * 1 = row-length buckets (grouped form: rows ordered by length) / 2 = one
+   * mapping, XCD remap on/off.  autotuned = 1
    * when the options left all three on auto and the autotuner chose them
    * (timed on the device at the run's width, cached per matrix and width). */
   int autotuned;

@@ -27,6 +27,10 @@ struct MapSetup {
   int nnzPerLane = 4;  // target nonzeros per lane: lanes per row = nextpow2(ceil(len / nnzPerLane))
   int rowMap = 1;      // 1: row-length buckets (rows permuted); 2: one mapping for all rows
   bool xcd = false;    // give each XCD a contiguous range of rows
+  // B = 16 or 32: one row per B-lane group, 64 / B rows per wavefront, the
+  // rows' entries read through the scalar cache (nnzPerLane unused; rowMap 1
+  // orders rows by length, 2 keeps their order).
+  bool group = false;
 };
 
 struct RunSetup {

@@ -77,26 +77,33 @@ struct CaseScaled {  // per case: norms of the original vectors and initial omeg
 struct TuneKey {
   uint64_t shape;
   int B, precision, device;
+  int maps;  // 0: autotuned mapping; else the options' explicit mapping
   bool operator<(const TuneKey& r) const {
-    return std::tie(shape, B, precision, device) < std::tie(r.shape, r.B, r.precision, r.device);
+    return std::tie(shape, B, precision, device, maps) < std::tie(r.shape, r.B, r.precision, r.device, r.maps);
   }
 };
 
-std::mutex gTuneMutex;
-std::map<TuneKey, std::pair<MapSetup, MapSetup>> gTuned;
+// A width's row mapping (column and row kernel) and the measured time of one
+// iteration (column + row kernel, ms) under it.
+struct Tuned {
+  MapSetup col, row;
+  double ms = 0;
+};
 
-bool tuned_maps(const TuneKey& k, MapSetup* col, MapSetup* row) {
+std::mutex gTuneMutex;
+std::map<TuneKey, Tuned> gTuned;
+
+bool tuned(const TuneKey& k, Tuned* t) {
   std::lock_guard<std::mutex> lk(gTuneMutex);
   auto it = gTuned.find(k);
   if (it == gTuned.end()) return false;
-  *col = it->second.first;
-  *row = it->second.second;
+  *t = it->second;
   return true;
 }
 
-void remember_maps(const TuneKey& k, const MapSetup& col, const MapSetup& row) {
+void remember(const TuneKey& k, const Tuned& t) {
   std::lock_guard<std::mutex> lk(gTuneMutex);
-  gTuned[k] = {col, row};
+  gTuned[k] = t;
 }
 
 // FNV-1a over the sparsity structure of the scaled A (shape, row pointers,
@@ -118,7 +125,8 @@ uint64_t structure_hash(const Prepared& P) {
 
 // Row mappings the autotuner times.  At B = 64 a wavefront serves one row
 // (lane = instance), so only the XCD remap is open; below, the lanes per row
-// (nnz-per-lane target, which sets the row tile S*B) and bucketing are too.
+// (nnz-per-lane target, which sets the row tile S*B) and bucketing are too,
+// and at B = 16 and 32 the grouped form (one row per B-lane group).
 std::vector<MapSetup> map_candidates(int B) {
   std::vector<MapSetup> c;
   for (bool xcd : {false, true}) {
@@ -128,6 +136,8 @@ std::vector<MapSetup> map_candidates(int B) {
     }
     for (int per : {2, 4, 8, 16})
       for (int map : {1, 2}) c.push_back(MapSetup{per, map, xcd});
+    if (B == 16 || B == 32)  // one row per B-lane group, rows by length or in order
+      for (int map : {1, 2}) c.push_back(MapSetup{4, map, xcd, true});
   }
   return c;
 }
@@ -251,44 +261,71 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
   std::vector<double> refKkt(k, kInf);
   std::vector<int> lastGain(k, 0);
   // Row mapping of the iteration kernels: the options' explicit choice, or
-  // (all three on auto) the autotuner's, cached per matrix structure, width,
-  // precision and device for the life of the process.
+  // (all three on auto) the autotuner's.  Either way the first run of a
+  // width times its mapping on the device (scratch state, before the data is
+  // uploaded), and mapping and time are cached per matrix structure, width,
+  // precision and device for the life of the process: the compaction below
+  // uses the times to pick widths.
   const bool autoMap = o.lanes_per_row == 0 && o.row_map == 0 && o.xcd_remap < 0;
-  const uint64_t shape = autoMap ? structure_hash(P) : 0;
+  const uint64_t shape = structure_hash(P);
+  const int mapsKey = autoMap ? 0 : 1 + o.lanes_per_row * 100 + o.row_map * 10 + (o.xcd_remap + 1);
   bool mapsRecorded = false, tunedFirst = false;
   MapSetup firstCol, firstRow;
   double tuneSeconds = 0;
-  auto make_run = [&](const std::vector<int>& ids, int B) {
+  auto fixed_map = [&](int B) {
+    MapSetup ms;
+    ms.nnzPerLane = o.lanes_per_row > 0 ? o.lanes_per_row : 4;
+    ms.rowMap = o.row_map == 2 ? 2 : 1;
+    ms.xcd = o.xcd_remap == 1 || (o.xcd_remap < 0 && B >= 16);
+    return ms;
+  };
+  // A run of width B (case data not yet uploaded) with its tuned mapping.
+  auto create_run = [&](int B, int prec) {
     RunSetup rs;
     rs.m = m;
     rs.n = n;
     rs.B = B;
-    rs.precision = runPrec == PDHG_FP64 ? PDHG_FP64 : PDHG_FP32;
+    rs.precision = prec == PDHG_FP64 ? PDHG_FP64 : PDHG_FP32;
     rs.sharedCost = shC;
     rs.sharedColBounds = shB;
     rs.sharedRowBounds = shR;
     rs.rho = o.reflection;
     rs.checkEvery = o.check_every;
     rs.useGraph = o.use_graph != 0;
-    for (MapSetup* ms : {&rs.colMap, &rs.rowMap}) {
-      ms->nnzPerLane = o.lanes_per_row > 0 ? o.lanes_per_row : 4;
-      ms->rowMap = o.row_map == 2 ? 2 : 1;
-      ms->xcd = o.xcd_remap == 1 || (o.xcd_remap < 0 && B >= 16);
-    }
-    const TuneKey key{shape, B, rs.precision, o.device};
-    const bool tuneNow = autoMap && !tuned_maps(key, &rs.colMap, &rs.rowMap);
+    const TuneKey key{shape, B, rs.precision, o.device, mapsKey};
+    Tuned t;
+    const bool known = tuned(key, &t);
+    rs.colMap = known ? t.col : fixed_map(B);
+    rs.rowMap = known ? t.row : fixed_map(B);
     auto run = DeviceRun::create(o.device, P, rs);
-    if (tuneNow) {
+    if (!known) {
       const auto t0 = Clock::now();
-      const std::vector<MapSetup> cand = map_candidates(B);
+      const std::vector<MapSetup> cand = autoMap ? map_candidates(B) : std::vector<MapSetup>{fixed_map(B)};
       int best[2];
       double ms[2];
       run->autotune(P, cand, 8, best, ms);
-      rs.colMap = cand[best[0]];
-      rs.rowMap = cand[best[1]];
-      remember_maps(key, rs.colMap, rs.rowMap);
+      t.col = cand[best[0]];
+      t.row = cand[best[1]];
+      t.ms = ms[0] + ms[1];
+      remember(key, t);
+      rs.colMap = t.col;
+      rs.rowMap = t.row;
       tuneSeconds += since_s(t0);
     }
+    return std::make_pair(std::move(run), rs);
+  };
+  // Time of one iteration at width W (probing the width once).
+  auto width_cost = [&](int W, int prec) {
+    const TuneKey key{shape, W, prec == PDHG_FP64 ? PDHG_FP64 : PDHG_FP32, o.device, mapsKey};
+    Tuned t;
+    if (!tuned(key, &t)) {
+      create_run(W, prec);
+      tuned(key, &t);
+    }
+    return t.ms;
+  };
+  auto make_run = [&](const std::vector<int>& ids, int B) {
+    auto [run, rs] = create_run(B, runPrec);
     if (!mapsRecorded) {
       mapsRecorded = true;
       tunedFirst = autoMap;
@@ -431,7 +468,24 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       // Re-pack: finished instances leave (compaction) and/or the run changes
       // precision; live instances keep their iterates, anchors and restart
       // state, moved through an FP64 host copy.
-      const bool compactNow = o.compact && B > 1 && remaining <= B / 2;
+      // Compaction: once at most half the lanes are live, move them to the
+      // width (a power of two holding them all) whose measured iteration is
+      // cheapest for this matrix and precision; a narrower run is not always
+      // cheaper (the row mapping changes with the width).
+      const int newPrec = toFp64 ? PDHG_FP64 : runPrec;
+      int newB = B;
+      if ((o.compact && B > 1 && remaining <= B / 2) || toFp64) {
+        double best = toFp64 ? kInf : width_cost(B, runPrec);
+        for (int W = next_pow2(remaining); W <= B; W *= 2) {
+          if (!o.compact && W != B) continue;
+          const double c = width_cost(W, newPrec);
+          if (c < best) {
+            best = c;
+            newB = W;
+          }
+        }
+      }
+      const bool compactNow = newB < B;
       if (compactNow || toFp64) {
         std::vector<int> fin, live;
         for (int b = 0; b < real; ++b) (done[b] ? fin : live).push_back(b);
@@ -449,7 +503,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
         std::vector<int> nids;
         for (int b : live) nids.push_back(ids[b]);
         ids = nids;
-        B = next_pow2(static_cast<int>(ids.size()));
+        B = newB;
         if (toFp64) {
           runPrec = PDHG_FP64;
           if (acc.fp64_switch_iteration < 0) acc.fp64_switch_iteration = it;
@@ -503,10 +557,10 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     std::snprintf(acc.backend, sizeof(acc.backend), "hip-gfx950");
     std::snprintf(acc.device_name, sizeof(acc.device_name), "%s", di.name.c_str());
     acc.autotuned = tunedFirst ? 1 : 0;
-    acc.col_nnz_per_lane = firstCol.nnzPerLane;
+    acc.col_nnz_per_lane = firstCol.group ? 0 : firstCol.nnzPerLane;
     acc.col_row_map = firstCol.rowMap;
     acc.col_xcd = firstCol.xcd ? 1 : 0;
-    acc.row_nnz_per_lane = firstRow.nnzPerLane;
+    acc.row_nnz_per_lane = firstRow.group ? 0 : firstRow.nnzPerLane;
     acc.row_row_map = firstRow.rowMap;
     acc.row_xcd = firstRow.xcd ? 1 : 0;
     acc.tune_seconds = tuneSeconds;

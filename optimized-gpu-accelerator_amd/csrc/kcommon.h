@@ -42,6 +42,7 @@ struct RowPlan {
   int logS[kMaxBuckets];           // log2 lanes per row (per instance)
   const int* perm;                 // row order, NULL = identity
   int xcd;                         // 1: contiguous row ranges per XCD
+  int grp;                         // 1: one row per B-lane group, entries via the scalar cache
 };
 
 // Workgroup ids are dispatched round-robin over the 8 XCDs; this bijection
@@ -82,40 +83,20 @@ __device__ inline T reduce_sub(T v, int logS) {
   return v;
 }
 
-// sum_e val[e] * x[idx[e]*B + b] over e = e0, e0+step, ... < e1, with four
-// independent chains so the index, value and gather loads of four nonzeros are
-// in flight together.
-template <int LOGB, typename Acc, typename V, typename X>
-__device__ inline Acc row_dot(const int* __restrict__ idx, const V* __restrict__ val, const X* __restrict__ x,
-                              int e0, int e1, int step, int b) {
-  Acc a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-  int e = e0;
-  for (; e + 3 * step < e1; e += 4 * step) {
-    const int c0 = idx[e], c1 = idx[e + step], c2 = idx[e + 2 * step], c3 = idx[e + 3 * step];
-    const Acc v0 = val[e], v1 = val[e + step], v2 = val[e + 2 * step], v3 = val[e + 3 * step];
-    a0 += v0 * static_cast<Acc>(x[(static_cast<size_t>(c0) << LOGB) + b]);
-    a1 += v1 * static_cast<Acc>(x[(static_cast<size_t>(c1) << LOGB) + b]);
-    a2 += v2 * static_cast<Acc>(x[(static_cast<size_t>(c2) << LOGB) + b]);
-    a3 += v3 * static_cast<Acc>(x[(static_cast<size_t>(c3) << LOGB) + b]);
-  }
-  for (; e < e1; e += step) a0 += static_cast<Acc>(val[e]) * static_cast<Acc>(x[(static_cast<size_t>(idx[e]) << LOGB) + b]);
-  return (a0 + a1) + (a2 + a3);
-}
-
 // Vector layout of a B-lane run (B = 1 << LOGB): element (i, b) sits at
-// i*B + b, except for FP64 at B = 64 ("split"): the 64 values of row i take
-// 128 dwords, the low words of lanes 0..63 followed by their high words.  A
-// 64-lane access is then two fully coalesced dword accesses; a 64-lane
-// dwordx2 access costs the vector L1 twice the cycles per byte.
+// i*B + b, except for FP64 at B >= 16 ("split"): the B values of row i take
+// 2B dwords, the low words of lanes 0..B-1 followed by their high words.  A
+// 64-lane access is then two dword accesses instead of one dwordx2 access,
+// which costs the vector L1 twice the cycles per byte.
 template <typename T, int LOGB>
-constexpr bool kSplit = sizeof(T) == 8 && LOGB == 6;
+constexpr bool kSplit = sizeof(T) == 8 && LOGB >= 4;
 
 template <bool SPLIT, int LOGB, typename T>
 __device__ inline T vld(const T* __restrict__ p, size_t i, int b) {
   if constexpr (SPLIT) {
-    static_assert(sizeof(T) == 8 && LOGB == 6, "split layout is FP64 at B = 64");
-    const unsigned* u = reinterpret_cast<const unsigned*>(p) + (i << 7);
-    const unsigned lo = u[b], hi = u[64 + b];
+    static_assert(sizeof(T) == 8, "split layout is FP64");
+    const unsigned* u = reinterpret_cast<const unsigned*>(p) + (i << (LOGB + 1));
+    const unsigned lo = u[b], hi = u[(1 << LOGB) + b];
     return __builtin_bit_cast(T, (static_cast<unsigned long long>(hi) << 32) | lo);
   } else {
     return p[(i << LOGB) + b];
@@ -125,14 +106,34 @@ __device__ inline T vld(const T* __restrict__ p, size_t i, int b) {
 template <bool SPLIT, int LOGB, typename T>
 __device__ inline void vst(T* __restrict__ p, size_t i, int b, T v) {
   if constexpr (SPLIT) {
-    static_assert(sizeof(T) == 8 && LOGB == 6, "split layout is FP64 at B = 64");
-    unsigned* u = reinterpret_cast<unsigned*>(p) + (i << 7);
+    static_assert(sizeof(T) == 8, "split layout is FP64");
+    unsigned* u = reinterpret_cast<unsigned*>(p) + (i << (LOGB + 1));
     const unsigned long long bits = __builtin_bit_cast(unsigned long long, v);
     u[b] = static_cast<unsigned>(bits);
-    u[64 + b] = static_cast<unsigned>(bits >> 32);
+    u[(1 << LOGB) + b] = static_cast<unsigned>(bits >> 32);
   } else {
     p[(i << LOGB) + b] = v;
   }
+}
+
+// sum_e val[e] * x[idx[e]*B + b] over e = e0, e0+step, ... < e1, with four
+// independent chains so the index, value and gather loads of four nonzeros are
+// in flight together.
+template <int LOGB, typename Acc, bool SPLIT = false, typename V, typename X>
+__device__ inline Acc row_dot(const int* __restrict__ idx, const V* __restrict__ val, const X* __restrict__ x,
+                              int e0, int e1, int step, int b) {
+  Acc a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int e = e0;
+  for (; e + 3 * step < e1; e += 4 * step) {
+    const int c0 = idx[e], c1 = idx[e + step], c2 = idx[e + 2 * step], c3 = idx[e + 3 * step];
+    const Acc v0 = val[e], v1 = val[e + step], v2 = val[e + 2 * step], v3 = val[e + 3 * step];
+    a0 += v0 * static_cast<Acc>(vld<SPLIT, LOGB>(x, c0, b));
+    a1 += v1 * static_cast<Acc>(vld<SPLIT, LOGB>(x, c1, b));
+    a2 += v2 * static_cast<Acc>(vld<SPLIT, LOGB>(x, c2, b));
+    a3 += v3 * static_cast<Acc>(vld<SPLIT, LOGB>(x, c3, b));
+  }
+  for (; e < e1; e += step) a0 += static_cast<Acc>(val[e]) * static_cast<Acc>(vld<SPLIT, LOGB>(x, idx[e], b));
+  return (a0 + a1) + (a2 + a3);
 }
 
 // B = 64: a whole wavefront serves one row (lane = instance), so the row's
@@ -181,6 +182,77 @@ __device__ inline Acc row_dot_wave(const int* __restrict__ idx, const V* __restr
   return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
+// Zero entries (column 0, weight 0) after the last nonzero of the CSR arrays
+// the iteration kernels read: whole-chunk scalar loads may run up to a
+// chunk past a row's end.
+constexpr int kCsrPad = 64;
+
+template <typename U>
+std::vector<U> csr_padded(const std::vector<U>& v) {
+  std::vector<U> out(v);
+  out.resize(v.size() + kCsrPad, U(0));
+  return out;
+}
+
+// B = 16 or 32, one row per B-lane group (G = 64 / B rows per wavefront):
+// each group's column indices and values come through the scalar cache a
+// chunk at a time, every lane takes its own group's entry with a select, and
+// the CH gathers of a chunk are issued back to back.  rows[g] is the wave's
+// row for group g (-1: none); a group whose row is done loads nothing and
+// gathers column 0 with weight 0.
+template <int LOGB, typename Acc, bool SPLIT, typename V, typename X>
+__device__ inline Acc row_dot_groups(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                     const V* __restrict__ val, const X* __restrict__ x, const int (&rows)[64 >> LOGB],
+                                     int b) {
+  constexpr int G = 64 >> LOGB;
+  constexpr int CH = G == 2 ? 8 : 4;
+  static_assert(CH <= kCsrPad, "CSR padding too short for the chunk");
+  const int g = (threadIdx.x & 63) >> LOGB;
+  int e0[G], len[G];
+  int maxLen = 0;
+#pragma unroll
+  for (int q = 0; q < G; ++q) {
+    const int r = rows[q];
+    e0[q] = r >= 0 ? __builtin_amdgcn_readfirstlane(ptr[r]) : 0;
+    len[q] = r >= 0 ? __builtin_amdgcn_readfirstlane(ptr[r + 1]) - e0[q] : 0;
+    maxLen = max(maxLen, len[q]);
+  }
+  Acc a0 = 0, a1 = 0;
+  for (int k = 0; k < maxLen; k += CH) {
+    int c[CH];
+    Acc v[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      c[u] = 0;
+      v[u] = 0;
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      if (k < len[q]) {
+        int cq[CH];
+        Acc vq[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          cq[u] = idx[e0[q] + k + u];
+          const Acc w = static_cast<Acc>(val[e0[q] + k + u]);
+          vq[u] = k + u < len[q] ? w : static_cast<Acc>(0);
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          c[u] = g == q ? cq[u] : c[u];
+          v[u] = g == q ? vq[u] : v[u];
+        }
+      }
+    }
+    Acc gth[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) gth[u] = static_cast<Acc>(vld<SPLIT, LOGB>(x, c[u], b));
+#pragma unroll
+    for (int u = 0; u < CH; ++u) (u & 1 ? a1 : a0) += v[u] * gth[u];
+  }
+  return a0 + a1;
+}
+
 // Row dot product for the lane layout of LOGB (the B = 64 case uses the
 // wave-uniform path above; there the plan always has one lane per instance).
 template <int LOGB, typename Acc, bool SPLIT = false, typename V, typename X>
@@ -189,8 +261,30 @@ __device__ inline Acc row_sum(const int* __restrict__ ptr, const int* __restrict
   if constexpr (LOGB == 6) {
     return row_dot_wave<Acc, SPLIT>(idx, val, x, ptr[s.row], ptr[s.row + 1], b);
   } else {
-    return row_dot<LOGB, Acc>(idx, val, x, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
+    return row_dot<LOGB, Acc, SPLIT>(idx, val, x, ptr[s.row] + s.sub, ptr[s.row + 1], 1 << s.logS, b);
   }
+}
+
+// The row dot product of lane b under plan p (block bid): the grouped form
+// when the plan asks for it, else row_sum (0 for a lane without a row).
+template <int LOGB, typename Acc, bool SPLIT = false, typename V, typename X>
+__device__ inline Acc row_sum_plan(const RowPlan& p, int bid, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                   const V* __restrict__ val, const X* __restrict__ x, const Slot& s, int b) {
+  if constexpr (LOGB == 4 || LOGB == 5) {
+    if (p.grp) {
+      constexpr int G = 64 >> LOGB;
+      const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+      const int base = ((bid - p.blkStart[0]) << (kLogWG - LOGB)) + wave * G;  // locate() with one lane per row
+      int rows[G];
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        const int r = base + q;
+        rows[q] = r < p.rowCount[0] ? (p.perm ? p.perm[p.rowStart[0] + r] : p.rowStart[0] + r) : -1;
+      }
+      return row_dot_groups<LOGB, Acc, SPLIT>(ptr, idx, val, x, rows, b);
+    }
+  }
+  return s.row >= 0 ? row_sum<LOGB, Acc, SPLIT>(ptr, idx, val, x, s, b) : static_cast<Acc>(0);
 }
 
 // Per-workgroup FP64 partial sums of NQ quantities for each instance

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict
   const bool rst = st.applyAt[b] == it;
   const T* __restrict__ ysrc = rst ? yp : y;
   T acc = 0;
-  if (s.row >= 0) acc = row_sum<LOGB, T, SPL>(ptr, idx, val, ysrc, s, b);
+  acc = row_sum_plan<LOGB, T, SPL>(plan, bid, ptr, idx, val, ysrc, s, b);
   acc = reduce_sub<LOGB>(acc, s.logS);
   double q[2] = {0.0, 0.0};
   if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
@@ -116,10 +116,8 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
   const int b = threadIdx.x & (B - 1);
   T acc = 0;
   double acc64 = 0;
-  if (s.row >= 0) {
-    acc = row_sum<LOGB, T, SPL>(ptr, idx, val, xp, s, b);
-    if (CHECK) acc64 = row_sum<LOGB, double, SPL>(ptr, idx, val64, xp, s, b);
-  }
+  acc = row_sum_plan<LOGB, T, SPL>(plan, bid, ptr, idx, val, xp, s, b);
+  if (CHECK) acc64 = row_sum_plan<LOGB, double, SPL>(plan, bid, ptr, idx, val64, xp, s, b);
   acc = reduce_sub<LOGB>(acc, s.logS);
   if (CHECK) acc64 = reduce_sub<LOGB>(acc64, s.logS);
   double q[4] = {0.0, 0.0, 0.0, 0.0};
@@ -189,7 +187,7 @@ __global__ __launch_bounds__(kWG) void k_colcheck(RowPlan plan, const int* __res
   const int b = threadIdx.x & (B - 1);
   constexpr bool SPL = kSplit<T, LOGB>;
   double acc = 0;
-  if (s.row >= 0) acc = row_sum<LOGB, double, SPL>(ptr, idx, val64, yp, s, b);
+  acc = row_sum_plan<LOGB, double, SPL>(plan, bid, ptr, idx, val64, yp, s, b);
   acc = reduce_sub<LOGB>(acc, s.logS);
   double q[3] = {0.0, 0.0, 0.0};
   if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
@@ -264,7 +262,7 @@ __global__ __launch_bounds__(kWG) void k_spmv(RowPlan plan, const int* __restric
   const Slot s = locate<LOGB>(plan, bid);
   const int b = threadIdx.x & (B - 1);
   T acc = 0;
-  if (s.row >= 0) acc = row_sum<LOGB, T, SPL>(ptr, idx, val, in, s, b);
+  acc = row_sum_plan<LOGB, T, SPL>(plan, bid, ptr, idx, val, in, s, b);
   acc = reduce_sub<LOGB>(acc, s.logS);
   if (s.row >= 0 && s.sub == 0) vst<SPL, LOGB>(out, s.row, b, acc);
 }
@@ -409,7 +407,22 @@ PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms) {
     return std::min(l, maxLogS);
   };
   int nb = 0;
-  if (ms.rowMap == 2 || maxLogS == 0) {
+  if (ms.group && (logB == 4 || logB == 5)) {
+    // One row per B-lane group; rowMap 1 orders the rows by length so the
+    // rows sharing a wavefront have similar lengths.
+    h.plan.logS[0] = 0;
+    h.plan.rowStart[0] = 0;
+    h.plan.rowCount[0] = rows;
+    h.plan.grp = 1;
+    if (ms.rowMap == 1) {
+      h.perm.resize(rows);
+      for (int i = 0; i < rows; ++i) h.perm[i] = i;
+      std::stable_sort(h.perm.begin(), h.perm.end(), [&](int a, int b) {
+        return M.ptr[a + 1] - M.ptr[a] > M.ptr[b + 1] - M.ptr[b];
+      });
+    }
+    nb = 1;
+  } else if (ms.rowMap == 2 || maxLogS == 0) {
     // One mapping: S from the mean row length.
     const double mean = rows > 0 ? static_cast<double>(M.nnz()) / rows : 0;
     const int l = logs_of(static_cast<int>(std::ceil(mean)));
@@ -567,22 +580,22 @@ class RunImpl final : public DeviceRun {
     B_ = s.B;
     logB_ = ilog2(B_);
     PDHG_REQUIRE((1 << logB_) == B_ && B_ <= 64, "batch width must be a power of two <= 64");
-    split_ = sizeof(T) == 8 && B_ == 64;
+    split_ = sizeof(T) == 8 && B_ >= 16;  // kSplit
     k_ = kern_table<T>(logB_);
     nnz_ = P.A.nnz();
     // Matrices: FP32 (or FP64) values for the iterations, FP64 for the checks.
     aPtr_ = upload_int(P.A.ptr);
-    aIdx_ = upload_int(P.A.idx);
+    aIdx_ = upload_int(csr_padded(P.A.idx));
     tPtr_ = upload_int(P.AT.ptr);
-    tIdx_ = upload_int(P.AT.idx);
-    aVal64_ = upload_dbl(P.A.val);
-    tVal64_ = upload_dbl(P.AT.val);
+    tIdx_ = upload_int(csr_padded(P.AT.idx));
+    aVal64_ = upload_dbl(csr_padded(P.A.val));
+    tVal64_ = upload_dbl(csr_padded(P.AT.val));
     if (sizeof(T) == 8) {
       aVal_ = reinterpret_cast<T*>(aVal64_);
       tVal_ = reinterpret_cast<T*>(tVal64_);
     } else {
-      aVal_ = upload_conv(P.A.val);
-      tVal_ = upload_conv(P.AT.val);
+      aVal_ = upload_conv(csr_padded(P.A.val));
+      tVal_ = upload_conv(csr_padded(P.AT.val));
     }
     rfac_ = upload_dbl(P.rfac);
     cfac_ = upload_dbl(P.cfac);
@@ -957,27 +970,27 @@ class RunImpl final : public DeviceRun {
     return d;
   }
   // Interleaved rows x B vectors between element order (i*B + b) and the
-  // device layout (the split layout for FP64 at B = 64, kSplit).
+  // device layout (the split layout for FP64 at B >= 16, kSplit).
   void to_device_layout(std::vector<T>& v) const {
     if (!split_) return;
-    uint32_t w[128];
-    for (size_t r = 0; r + 64 <= v.size(); r += 64) {
-      for (int b = 0; b < 64; ++b) {
+    std::vector<uint32_t> w(2 * static_cast<size_t>(B_));
+    for (size_t r = 0; r + B_ <= v.size(); r += B_) {
+      for (int b = 0; b < B_; ++b) {
         uint64_t bits;
         std::memcpy(&bits, &v[r + b], 8);
         w[b] = static_cast<uint32_t>(bits);
-        w[64 + b] = static_cast<uint32_t>(bits >> 32);
+        w[B_ + b] = static_cast<uint32_t>(bits >> 32);
       }
-      std::memcpy(&v[r], w, sizeof w);
+      std::memcpy(&v[r], w.data(), w.size() * sizeof(uint32_t));
     }
   }
   void from_device_layout(std::vector<T>& v) const {
     if (!split_) return;
-    uint32_t w[128];
-    for (size_t r = 0; r + 64 <= v.size(); r += 64) {
-      std::memcpy(w, &v[r], sizeof w);
-      for (int b = 0; b < 64; ++b) {
-        const uint64_t bits = (static_cast<uint64_t>(w[64 + b]) << 32) | w[b];
+    std::vector<uint32_t> w(2 * static_cast<size_t>(B_));
+    for (size_t r = 0; r + B_ <= v.size(); r += B_) {
+      std::memcpy(w.data(), &v[r], w.size() * sizeof(uint32_t));
+      for (int b = 0; b < B_; ++b) {
+        const uint64_t bits = (static_cast<uint64_t>(w[B_ + b]) << 32) | w[b];
         std::memcpy(&v[r + b], &bits, 8);
       }
     }
