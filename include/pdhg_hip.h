@@ -142,6 +142,7 @@ typedef struct pdhg_stats {
   int col_nnz_per_lane, col_row_map, col_xcd;
   int row_nnz_per_lane, row_row_map, row_xcd;
   double tune_seconds;
+  int col_rows_per_wave, row_rows_per_wave; /* B = 64: 1, or 2 rows per wavefront */
 } pdhg_stats;
 
 /* PdhgResult (src/pdhg/pdhg.h:58-65).  x (n) and row_dual (m) are

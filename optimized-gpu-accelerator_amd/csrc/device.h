@@ -31,6 +31,8 @@ struct MapSetup {
   // rows' entries read through the scalar cache (nnzPerLane unused; rowMap 1
   // orders rows by length, 2 keeps their order).
   bool group = false;
+  // B = 64: rows per wavefront (1, or 2 with their chunks interleaved).
+  int rowsPerWave = 1;
 };
 
 struct RunSetup {

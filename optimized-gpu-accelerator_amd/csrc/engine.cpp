@@ -131,7 +131,7 @@ std::vector<MapSetup> map_candidates(int B) {
   std::vector<MapSetup> c;
   for (bool xcd : {false, true}) {
     if (B == 64) {
-      c.push_back(MapSetup{4, 1, xcd});
+      for (int rpw : {1, 2}) c.push_back(MapSetup{4, 1, xcd, false, rpw});
       continue;
     }
     for (int per : {2, 4, 8, 16})
@@ -564,6 +564,8 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     acc.row_row_map = firstRow.rowMap;
     acc.row_xcd = firstRow.xcd ? 1 : 0;
     acc.tune_seconds = tuneSeconds;
+    acc.col_rows_per_wave = firstCol.rowsPerWave;
+    acc.row_rows_per_wave = firstRow.rowsPerWave;
     *stats = acc;
   }
 }

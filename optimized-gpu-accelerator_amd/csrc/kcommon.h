@@ -43,6 +43,7 @@ struct RowPlan {
   const int* perm;                 // row order, NULL = identity
   int xcd;                         // 1: contiguous row ranges per XCD
   int grp;                         // 1: one row per B-lane group, entries via the scalar cache
+  int rpw;                         // B = 64: rows per wavefront (1 or 2)
 };
 
 // Workgroup ids are dispatched round-robin over the 8 XCDs; this bijection
@@ -253,6 +254,49 @@ __device__ inline Acc row_dot_groups(const int* __restrict__ ptr, const int* __r
   return a0 + a1;
 }
 
+// B = 64, two rows per wavefront: row_dot_wave for rows [a0, a1) and
+// [c0, c1) with their chunks interleaved, so each wave keeps two chunks of
+// gathers in flight.  A row that has run out reads the other row's entries
+// with zero weights (at least one of them has entries in every round).
+template <typename Acc, bool SPLIT, typename V, typename X>
+__device__ inline void row_dot_wave2(const int* __restrict__ idx, const V* __restrict__ val, const X* __restrict__ x,
+                                     int a0, int a1, int c0, int c1, int b, Acc& outA, Acc& outC) {
+  constexpr int CH = 8;
+  a0 = __builtin_amdgcn_readfirstlane(a0);
+  a1 = __builtin_amdgcn_readfirstlane(a1);
+  c0 = __builtin_amdgcn_readfirstlane(c0);
+  c1 = __builtin_amdgcn_readfirstlane(c1);
+  Acc sa[2] = {0, 0}, sc[2] = {0, 0};
+  for (int ea = a0, ec = c0; ea < a1 || ec < c1; ea += CH, ec += CH) {
+    const bool ha = ea < a1, hc = ec < c1;
+    const int fa = ha ? ea : ec, la = ha ? a1 - 1 : c1 - 1;  // entries read for row a
+    const int fc = hc ? ec : ea, lc = hc ? c1 - 1 : a1 - 1;  // entries read for row c
+    int ca[CH], cc[CH];
+    Acc va[CH], vc[CH], ga[CH], gc[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int pa = min(fa + u, la), pc = min(fc + u, lc);
+      ca[u] = idx[pa];
+      cc[u] = idx[pc];
+      const Acc wa = static_cast<Acc>(val[pa]), wc = static_cast<Acc>(val[pc]);
+      va[u] = ha && ea + u < a1 ? wa : static_cast<Acc>(0);
+      vc[u] = hc && ec + u < c1 ? wc : static_cast<Acc>(0);
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      ga[u] = static_cast<Acc>(vld<SPLIT, 6>(x, ca[u], b));
+      gc[u] = static_cast<Acc>(vld<SPLIT, 6>(x, cc[u], b));
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      sa[u & 1] += va[u] * ga[u];
+      sc[u & 1] += vc[u] * gc[u];
+    }
+  }
+  outA = sa[0] + sa[1];
+  outC = sc[0] + sc[1];
+}
+
 // Row dot product for the lane layout of LOGB (the B = 64 case uses the
 // wave-uniform path above; there the plan always has one lane per instance).
 template <int LOGB, typename Acc, bool SPLIT = false, typename V, typename X>
@@ -285,6 +329,31 @@ __device__ inline Acc row_sum_plan(const RowPlan& p, int bid, const int* __restr
     }
   }
   return s.row >= 0 ? row_sum<LOGB, Acc, SPLIT>(ptr, idx, val, x, s, b) : static_cast<Acc>(0);
+}
+
+// The rows of lane (b, sub) under plan p and their (sub-lane reduced) dot
+// products: one row (rows[1] = -1), or at B = 64 with two rows per
+// wavefront the wave's two rows, their chunks interleaved.
+template <int LOGB, int RPW, typename Acc, bool SPLIT = false, typename V, typename X>
+__device__ inline void lane_rows(const RowPlan& p, int bid, const Slot& s, const int* __restrict__ ptr,
+                                 const int* __restrict__ idx, const V* __restrict__ val, const X* __restrict__ x, int b,
+                                 int (&rows)[2], Acc (&acc)[2]) {
+  if constexpr (LOGB == 6 && RPW == 2) {
+    {  // one bucket, rows in order (no permutation at B = 64)
+      const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+      const int r0 = ((bid - p.blkStart[0]) * kWaves + wave) * 2;
+      rows[0] = r0 < p.rowCount[0] ? r0 : -1;
+      rows[1] = r0 + 1 < p.rowCount[0] ? r0 + 1 : -1;
+      const int a0 = rows[0] >= 0 ? ptr[rows[0]] : 0, a1 = rows[0] >= 0 ? ptr[rows[0] + 1] : 0;
+      const int c0 = rows[1] >= 0 ? ptr[rows[1]] : 0, c1 = rows[1] >= 0 ? ptr[rows[1] + 1] : 0;
+      row_dot_wave2<Acc, SPLIT>(idx, val, x, a0, a1, c0, c1, b, acc[0], acc[1]);
+      return;
+    }
+  }
+  rows[0] = s.row;
+  rows[1] = -1;
+  acc[0] = reduce_sub<LOGB>(row_sum_plan<LOGB, Acc, SPLIT>(p, bid, ptr, idx, val, x, s, b), s.logS);
+  acc[1] = 0;
 }
 
 // Per-workgroup FP64 partial sums of NQ quantities for each instance

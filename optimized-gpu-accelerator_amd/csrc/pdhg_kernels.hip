@@ -44,7 +44,7 @@ namespace {
 // 116-125, 138-152): x+ = clamp(x - tau (c - A^T y), lo, up), then
 // x <- lam((1+rho) x+ - rho x) + (1-lam) x0 with lam = (k+1)/(k+2).
 // CHECK adds ||x - x+||^2 and ||x+ - x0||^2 (quantities 0, 1).
-template <typename T, int LOGB, bool CHECK>
+template <typename T, int LOGB, bool CHECK, int RPW = 1>
 __global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict__ ptr, const int* __restrict__ idx,
                                              const T* __restrict__ val, const T* __restrict__ y,
                                              const T* __restrict__ yp, T* __restrict__ x, T* __restrict__ xp,
@@ -61,12 +61,14 @@ __global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict
   // of this iteration reads y+ (reference pdhg.cpp:160-162 then :589).
   const bool rst = st.applyAt[b] == it;
   const T* __restrict__ ysrc = rst ? yp : y;
-  T acc = 0;
-  acc = row_sum_plan<LOGB, T, SPL>(plan, bid, ptr, idx, val, ysrc, s, b);
-  acc = reduce_sub<LOGB>(acc, s.logS);
+  int rows[2];
+  T accs[2];
+  lane_rows<LOGB, RPW, T, SPL>(plan, bid, s, ptr, idx, val, ysrc, b, rows, accs);
   double q[2] = {0.0, 0.0};
-  if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
-    const int j = s.row;
+  for (int rr = 0; rr < 2; ++rr) {
+    if (rows[rr] < 0 || s.sub != 0 || st.done[b]) continue;
+    const int j = rows[rr];
+    const T acc = accs[rr];
     const T xin = vld<SPL, LOGB>(rst ? xp : x, j, b);
     T xo;
     if (rst) {
@@ -87,8 +89,8 @@ __global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict
     if (CHECK) {
       const double d = static_cast<double>(xin) - static_cast<double>(v);
       const double e = static_cast<double>(v) - static_cast<double>(xo);
-      q[0] = d * d;
-      q[1] = e * e;
+      q[0] += d * d;
+      q[1] += e * e;
     }
   }
   if (CHECK) block_partials<LOGB, 2>(q, part, 0, nblk, bid);
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict
 // CHECK also stores y+ and ax+, recomputes A x+ with FP64 values and FP64
 // accumulation, and adds (quantities 0..3): sum res^2, the row part of the
 // dual objective, ||y - y+||^2, ||y+ - y0||^2.
-template <typename T, int LOGB, bool CHECK>
+template <typename T, int LOGB, bool CHECK, int RPW = 1>
 __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict__ ptr, const int* __restrict__ idx,
                                              const T* __restrict__ val, const double* __restrict__ val64,
                                              const T* __restrict__ xp, T* __restrict__ y, T* __restrict__ yp,
@@ -114,15 +116,17 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
   const int bid = block_id(plan);
   const Slot s = locate<LOGB>(plan, bid);
   const int b = threadIdx.x & (B - 1);
-  T acc = 0;
-  double acc64 = 0;
-  acc = row_sum_plan<LOGB, T, SPL>(plan, bid, ptr, idx, val, xp, s, b);
-  if (CHECK) acc64 = row_sum_plan<LOGB, double, SPL>(plan, bid, ptr, idx, val64, xp, s, b);
-  acc = reduce_sub<LOGB>(acc, s.logS);
-  if (CHECK) acc64 = reduce_sub<LOGB>(acc64, s.logS);
+  int rows[2];
+  T accs[2];
+  double accs64[2] = {0.0, 0.0};
+  lane_rows<LOGB, RPW, T, SPL>(plan, bid, s, ptr, idx, val, xp, b, rows, accs);
+  if (CHECK) lane_rows<LOGB, RPW, double, SPL>(plan, bid, s, ptr, idx, val64, xp, b, rows, accs64);
   double q[4] = {0.0, 0.0, 0.0, 0.0};
-  if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
-    const int i = s.row;
+  for (int rr = 0; rr < 2; ++rr) {
+    if (rows[rr] < 0 || s.sub != 0 || st.done[b]) continue;
+    const int i = rows[rr];
+    const T acc = accs[rr];
+    const double acc64 = accs64[rr];
     const long long it = st.iter[b] + tloc;
     const bool rst = st.applyAt[b] == it;
     const T yin = vld<SPL, LOGB>(rst ? yp : y, i, b);
@@ -159,12 +163,12 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
       const double L = rl64[lr.at(i, b)], U = ru64[lr.at(i, b)];
       const double res = (fmax(L - acc64, 0.0) + fmax(acc64 - U, 0.0)) * rfac[i];
       const double yy = static_cast<double>(yplus);
-      q[0] = res * res;
-      if (yy > 0 && L > -1e300) q[1] = yy * L;
-      else if (yy < 0 && U < 1e300) q[1] = yy * U;
+      q[0] += res * res;
+      if (yy > 0 && L > -1e300) q[1] += yy * L;
+      else if (yy < 0 && U < 1e300) q[1] += yy * U;
       const double d = static_cast<double>(yin) - yy, e = yy - static_cast<double>(yo);
-      q[2] = d * d;
-      q[3] = e * e;
+      q[2] += d * d;
+      q[3] += e * e;
     }
   }
   if (CHECK) block_partials<LOGB, 4>(q, part, 0, nblk, bid);
@@ -173,7 +177,7 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
 // Column KKT terms at (x+, y+) in FP64 (reference pdhg.cpp:181-196):
 // d = c - A^T y+ ; dual residual (by bound pattern) * cfac, c'x+, and the
 // column part of the dual objective (quantities 2, 3, 4).
-template <typename T, int LOGB>
+template <typename T, int LOGB, int RPW = 1>
 __global__ __launch_bounds__(kWG) void k_colcheck(RowPlan plan, const int* __restrict__ ptr,
                                                   const int* __restrict__ idx, const double* __restrict__ val64,
                                                   const T* __restrict__ yp, const T* __restrict__ xp,
@@ -186,12 +190,14 @@ __global__ __launch_bounds__(kWG) void k_colcheck(RowPlan plan, const int* __res
   const Slot s = locate<LOGB>(plan, bid);
   const int b = threadIdx.x & (B - 1);
   constexpr bool SPL = kSplit<T, LOGB>;
-  double acc = 0;
-  acc = row_sum_plan<LOGB, double, SPL>(plan, bid, ptr, idx, val64, yp, s, b);
-  acc = reduce_sub<LOGB>(acc, s.logS);
+  int rows[2];
+  double accs[2];
+  lane_rows<LOGB, RPW, double, SPL>(plan, bid, s, ptr, idx, val64, yp, b, rows, accs);
   double q[3] = {0.0, 0.0, 0.0};
-  if (s.row >= 0 && s.sub == 0 && !st.done[b]) {
-    const int j = s.row;
+  for (int rr = 0; rr < 2; ++rr) {
+    if (rows[rr] < 0 || s.sub != 0 || st.done[b]) continue;
+    const int j = rows[rr];
+    const double acc = accs[rr];
     const double cj = c64[lc.at(j, b)], l = lo64[lb.at(j, b)], u = up64[lb.at(j, b)];
     const double d = cj - acc;
     const bool hl = l > -1e300, hu = u < 1e300;
@@ -200,10 +206,10 @@ __global__ __launch_bounds__(kWG) void k_colcheck(RowPlan plan, const int* __res
     else if (!hl) res = fmax(d, 0.0);
     else if (!hu) res = fmax(-d, 0.0);
     res *= cfac[j];
-    q[0] = res * res;
-    q[1] = cj * static_cast<double>(vld<SPL, LOGB>(xp, j, b));
-    if (d > 0 && hl) q[2] = d * l;
-    else if (d < 0 && hu) q[2] = d * u;
+    q[0] += res * res;
+    q[1] += cj * static_cast<double>(vld<SPL, LOGB>(xp, j, b));
+    if (d > 0 && hl) q[2] += d * l;
+    else if (d < 0 && hu) q[2] += d * u;
   }
   block_partials<LOGB, 3>(q, part, 2, nblk, bid);
 }
@@ -253,7 +259,7 @@ __global__ __launch_bounds__(kWG) void k_finalize(Inst st, const double* __restr
 // ---------------------------------------------------------------------------
 // Op-level kernels (the Backend vtable, reference pdhg.cpp:51-70)
 // ---------------------------------------------------------------------------
-template <typename T, int LOGB, bool SPL>
+template <typename T, int LOGB, bool SPL, int RPW = 1>
 __global__ __launch_bounds__(kWG) void k_spmv(RowPlan plan, const int* __restrict__ ptr, const int* __restrict__ idx,
                                               const T* __restrict__ val, const T* __restrict__ in,
                                               T* __restrict__ out) {
@@ -261,10 +267,11 @@ __global__ __launch_bounds__(kWG) void k_spmv(RowPlan plan, const int* __restric
   const int bid = block_id(plan);
   const Slot s = locate<LOGB>(plan, bid);
   const int b = threadIdx.x & (B - 1);
-  T acc = 0;
-  acc = row_sum_plan<LOGB, T, SPL>(plan, bid, ptr, idx, val, in, s, b);
-  acc = reduce_sub<LOGB>(acc, s.logS);
-  if (s.row >= 0 && s.sub == 0) vst<SPL, LOGB>(out, s.row, b, acc);
+  int rows[2];
+  T accs[2];
+  lane_rows<LOGB, RPW, T, SPL>(plan, bid, s, ptr, idx, val, in, b, rows, accs);
+  for (int rr = 0; rr < 2; ++rr)
+    if (rows[rr] >= 0 && s.sub == 0) vst<SPL, LOGB>(out, rows[rr], b, accs[rr]);
 }
 
 template <typename T>
@@ -460,10 +467,11 @@ PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms) {
     }
   }
   h.plan.nb = nb;
+  h.plan.rpw = logB == 6 && ms.rowsPerWave == 2 ? 2 : 1;
   int blk = 0;
   for (int q = 0; q < nb; ++q) {
     h.plan.blkStart[q] = blk;
-    const int rowsPerWg = kWG >> (logB + h.plan.logS[q]);
+    const int rowsPerWg = (kWG >> (logB + h.plan.logS[q])) * h.plan.rpw;
     blk += (h.plan.rowCount[q] + rowsPerWg - 1) / rowsPerWg;
   }
   h.plan.blkStart[nb] = blk;
@@ -533,22 +541,30 @@ struct Kern {
   using ColCheckFn = void (*)(RowPlan, const int*, const int*, const double*, const T*, const T*, const double*, Ld,
                               const double*, const double*, Ld, const double*, Inst, double*, int);
   using SpmvFn = void (*)(RowPlan, const int*, const int*, const T*, const T*, T*);
-  ColFn col, colChk;
-  RowFn row, rowChk;
-  ColCheckFn colcheck;
-  SpmvFn spmv;     // the run's vector layout
+  // [rpw - 1]: one row per wavefront, or (B = 64) two with their chunks interleaved
+  ColFn col[2], colChk[2];
+  RowFn row[2], rowChk[2];
+  ColCheckFn colcheck[2];
+  SpmvFn spmv[2];  // the run's vector layout
   SpmvFn spmvStd;  // element (i, b) at i*B + b (op-level backend)
 };
 
 template <typename T, int LOGB>
 Kern<T> kern_for() {
   Kern<T> k;
-  k.col = k_col<T, LOGB, false>;
-  k.colChk = k_col<T, LOGB, true>;
-  k.row = k_row<T, LOGB, false>;
-  k.rowChk = k_row<T, LOGB, true>;
-  k.colcheck = k_colcheck<T, LOGB>;
-  k.spmv = k_spmv<T, LOGB, kSplit<T, LOGB>>;
+  constexpr int R2 = LOGB == 6 ? 2 : 1;
+  k.col[0] = k_col<T, LOGB, false>;
+  k.colChk[0] = k_col<T, LOGB, true>;
+  k.row[0] = k_row<T, LOGB, false>;
+  k.rowChk[0] = k_row<T, LOGB, true>;
+  k.colcheck[0] = k_colcheck<T, LOGB>;
+  k.spmv[0] = k_spmv<T, LOGB, kSplit<T, LOGB>>;
+  k.col[1] = k_col<T, LOGB, false, R2>;
+  k.colChk[1] = k_col<T, LOGB, true, R2>;
+  k.row[1] = k_row<T, LOGB, false, R2>;
+  k.rowChk[1] = k_row<T, LOGB, true, R2>;
+  k.colcheck[1] = k_colcheck<T, LOGB, R2>;
+  k.spmv[1] = k_spmv<T, LOGB, kSplit<T, LOGB>, R2>;
   k.spmvStd = k_spmv<T, LOGB, false>;
   return k;
 }
@@ -1009,18 +1025,20 @@ class RunImpl final : public DeviceRun {
 
   void launch_spmv(bool transpose, const T* in, T* out) {
     const PlanHost& p = transpose ? colPlan_ : rowPlan_;
-    hipLaunchKernelGGL(k_.spmv, dim3(p.blocks), dim3(kWG), 0, stream_, p.plan, transpose ? tPtr_ : aPtr_,
+    hipLaunchKernelGGL(k_.spmv[p.plan.rpw - 1], dim3(p.blocks), dim3(kWG), 0, stream_, p.plan, transpose ? tPtr_ : aPtr_,
                        transpose ? tIdx_ : aIdx_, transpose ? tVal_ : aVal_, in, out);
     HIP_OK(hipGetLastError());
     launches++;
   }
   void launch_col(bool check, int tloc) {
-    hipLaunchKernelGGL(check ? k_.colChk : k_.col, dim3(colPlan_.blocks), dim3(kWG), 0, stream_, colPlan_.plan, tPtr_,
+    const int cr = colPlan_.plan.rpw - 1;
+    hipLaunchKernelGGL(check ? k_.colChk[cr] : k_.col[cr], dim3(colPlan_.blocks), dim3(kWG), 0, stream_, colPlan_.plan, tPtr_,
                        tIdx_, tVal_, y_, yp_, x_, xp_, x0_, c_, lc_, lo_, up_, lb_, st_, tloc, s_.rho, colPart_,
                        colPlan_.blocks);
   }
   void launch_row(bool check, int tloc) {
-    hipLaunchKernelGGL(check ? k_.rowChk : k_.row, dim3(rowPlan_.blocks), dim3(kWG), 0, stream_, rowPlan_.plan, aPtr_,
+    const int rr = rowPlan_.plan.rpw - 1;
+    hipLaunchKernelGGL(check ? k_.rowChk[rr] : k_.row[rr], dim3(rowPlan_.blocks), dim3(kWG), 0, stream_, rowPlan_.plan, aPtr_,
                        aIdx_, aVal_, aVal64_, xp_, y_, yp_, y0_, ax_, axp_, ax0_, rl_, ru_, rl64_, ru64_, lr_, rfac_,
                        st_, tloc, s_.rho, rowPart_, rowPlan_.blocks);
   }
@@ -1030,12 +1048,17 @@ class RunImpl final : public DeviceRun {
       launch_col(chk, t);
       launch_row(chk, t);
     }
-    hipLaunchKernelGGL(k_.colcheck, dim3(colPlan_.blocks), dim3(kWG), 0, stream_, colPlan_.plan, tPtr_, tIdx_,
+    hipLaunchKernelGGL(k_.colcheck[colPlan_.plan.rpw - 1], dim3(colPlan_.blocks), dim3(kWG), 0, stream_, colPlan_.plan, tPtr_, tIdx_,
                        tVal64_, yp_, xp_, c64_, lc_, lo64_, up64_, lb_, cfac_, st_, colPart_, colPlan_.blocks);
     hipLaunchKernelGGL(k_finalize, dim3(B_), dim3(kWG), 0, stream_, st_, colPart_, colPlan_.blocks, rowPart_,
                        rowPlan_.blocks, sc_, iters, s_.checkEvery);
     HIP_OK(hipGetLastError());
     launches += 2 * iters + 2;
+    // FP32 iterates: the activity ax follows the Halpern recursion of x
+    // (equal to A x in exact arithmetic), whose rounding drifts over long
+    // stretches without a restart; once per block it is recomputed as A x.
+    // FP64 keeps the recursion alone (the reference's arithmetic).
+    if (sizeof(T) == 4) launch_spmv(false, x_, ax_);
   }
   void capture(int iters) {
     HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));

@@ -63,7 +63,7 @@ class Stats(C.Structure):
                 ("backend", C.c_char * 32), ("device_name", C.c_char * 64),
                 ("autotuned", C.c_int), ("col_nnz_per_lane", C.c_int), ("col_row_map", C.c_int), ("col_xcd", C.c_int),
                 ("row_nnz_per_lane", C.c_int), ("row_row_map", C.c_int), ("row_xcd", C.c_int),
-                ("tune_seconds", C.c_double)]
+                ("tune_seconds", C.c_double), ("col_rows_per_wave", C.c_int), ("row_rows_per_wave", C.c_int)]
 
 
 class ResultStruct(C.Structure):
