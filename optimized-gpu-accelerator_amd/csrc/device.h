@@ -98,6 +98,17 @@ class DeviceRun {
   // Overwrites iterates and per-instance state of lanes 0..in.count-1 (after
   // upload() and init_state() of this run).
   virtual void import_lanes(const LaneState& in) = 0;
+  // Continuous batching: starts new cases in lanes `lanes` (each finished or
+  // free) while the other lanes keep running.  Arrays are interleaved over
+  // the K = lanes.size() new cases (element (i, q) at i*K + q), in the same
+  // scaled space as upload(); case arrays flagged shared in RunSetup are
+  // NULL.  The lanes get ax = A x, fresh anchors and restart state, and
+  // their iteration counters start at 0.
+  virtual void load_lanes(const std::vector<int>& lanes, const double* c, const double* lo, const double* up,
+                          const double* rl, const double* ru, const double* x, const double* y, const double* omega,
+                          const double* cNorm, const double* rhsNorm) = 0;
+  // Stops lanes (their kernels skip them from the next iteration on).
+  virtual void retire_lanes(const std::vector<int>& lanes) = 0;
   // Autotuner: times the column and the row iteration kernel under each
   // candidate row mapping (`reps` launches each, HIP events on the run's
   // stream) and keeps the fastest per direction.  Runs on scratch state, so

@@ -349,10 +349,19 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
   double solveSeconds = 0, kernelSeconds = 0, transfer = 0, setupSeconds = 0;
   bool timed = false;
 
-  for (int first = 0; first < k; first += 64) {
+  // More cases than one run holds: continuous batching.  A run of width 64
+  // starts with the first 64 cases; a lane whose case finishes (or reaches
+  // its own iteration limit) is given the next pending case while the other
+  // lanes keep running, so the run's time follows the cases' mean iteration
+  // count rather than the slowest case of every group of 64.
+  const bool stream = k > 64;
+  for (int first = 0; first < k; first += stream ? k : 64) {
     const auto tChunk = Clock::now();
     std::vector<int> ids;  // lane -> case
     for (int q = first; q < std::min(k, first + 64); ++q) ids.push_back(q);
+    std::vector<int> pending;  // cases not yet started (streaming)
+    if (stream)
+      for (int q = k - 1; q >= 64; --q) pending.push_back(q);
     int B = next_pow2(static_cast<int>(ids.size()));
     auto run = make_run(ids, B);
     transfer += since_s(tChunk);
@@ -376,6 +385,8 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     std::vector<int> done(B), restarts(B);
     std::vector<long long> doneIter(B);
     std::vector<char> extracted(k, 0);
+    std::vector<long long> loadIt(B, 0);  // run iteration at which each lane's case started
+    std::vector<char> limitHit(B, 0);      // lane reached its own iteration limit
     long long it = 0;
     int checks = 0;
     bool expired = false;
@@ -392,13 +403,14 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
         const int q = ids[b];
         pdhg_result& r = results[q];
         const CheckRecord& cr = rec[b];
-        if (done[b]) {
+        if (done[b] && !limitHit[b]) {
           r.status = PDHG_OPTIMAL;
           r.iterations = doneIter[b];
         } else {
-          PDHG_REQUIRE(final, "internal: extracting a live instance");
+          PDHG_REQUIRE(final || limitHit[b], "internal: extracting a live instance");
           r.status = expired ? PDHG_TIME_LIMIT : PDHG_ITERATION_LIMIT;
-          r.iterations = expired ? it - 1 : it;
+          const long long own = it - loadIt[b];
+          r.iterations = expired ? own - 1 : own;
         }
         r.restarts = restarts[b];
         r.objective = cr.pobj;
@@ -417,12 +429,58 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       }
     };
 
-    while (it < o.max_iterations) {
-      const int nIt = static_cast<int>(std::min<long long>(o.check_every, o.max_iterations - it));
+    while (stream || it < o.max_iterations) {
+      long long nextLimit = o.max_iterations;  // fewest iterations any live lane has left
+      for (int b = 0; b < static_cast<int>(ids.size()); ++b)
+        if (!done[b]) nextLimit = std::min(nextLimit, loadIt[b] + o.max_iterations - it);
+      const int nIt = static_cast<int>(std::max<long long>(1, std::min<long long>(o.check_every, nextLimit)));
       run->run_block(nIt);
       run->fetch(rec.data(), done.data(), doneIter.data(), restarts.data());
       it += nIt;
       ++checks;
+      if (stream) {
+        // Lanes at their own iteration limit stop (reported as such).
+        std::vector<int> capped;
+        for (int b = 0; b < static_cast<int>(ids.size()); ++b)
+          if (!done[b] && it - loadIt[b] >= o.max_iterations) {
+            capped.push_back(b);
+            limitHit[b] = 1;
+            done[b] = 1;
+          }
+        run->retire_lanes(capped);
+        // Refill: finished lanes take pending cases, a batch at a time.
+        std::vector<int> freeLanes, newly;
+        for (int b = 0; b < B; ++b) {
+          const bool used = b < static_cast<int>(ids.size());
+          if (used && !done[b]) continue;
+          if (used && !extracted[ids[b]]) newly.push_back(b);
+          freeLanes.push_back(b);
+        }
+        const int take = std::min<int>(static_cast<int>(freeLanes.size()), static_cast<int>(pending.size()));
+        if (take > 0 && (take >= 8 || take == static_cast<int>(pending.size()))) {
+          extract(newly, false);
+          std::vector<int> lanes(freeLanes.begin(), freeLanes.begin() + take), newIds;
+          for (int q = 0; q < take; ++q) {
+            newIds.push_back(pending.back());
+            pending.pop_back();
+          }
+          const LaneData d = lane_data(newIds, take);
+          run->load_lanes(lanes, shC ? nullptr : d.c.data(), shB ? nullptr : d.lo.data(),
+                          shB ? nullptr : d.up.data(), shR ? nullptr : d.rl.data(), shR ? nullptr : d.ru.data(),
+                          d.x.data(), d.y.data(), d.omega.data(), d.cNorm.data(), d.rhsNorm.data());
+          for (int q = 0; q < take; ++q) {
+            const int b = lanes[q];
+            if (b >= static_cast<int>(ids.size())) ids.resize(b + 1, newIds[q]);
+            ids[b] = newIds[q];
+            lastGain[newIds[q]] = checks;  // FP32_AUTO stall clock starts at the load
+            loadIt[b] = it;
+            limitHit[b] = 0;
+            done[b] = 0;
+            doneIter[b] = -1;
+            restarts[b] = 0;
+          }
+        }
+      }
       const int real = static_cast<int>(ids.size());
       int remaining = 0;
       for (int b = 0; b < real; ++b) remaining += done[b] ? 0 : 1;
@@ -474,10 +532,10 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       // cheaper (the row mapping changes with the width).
       const int newPrec = toFp64 ? PDHG_FP64 : runPrec;
       int newB = B;
-      if ((o.compact && B > 1 && remaining <= B / 2) || toFp64) {
+      if ((o.compact && B > 1 && remaining <= B / 2 && pending.empty()) || toFp64) {
         double best = toFp64 ? kInf : width_cost(B, runPrec);
         for (int W = next_pow2(remaining); W <= B; W *= 2) {
-          if (!o.compact && W != B) continue;
+          if ((!o.compact || !pending.empty()) && W != B) continue;
           const double c = width_cost(W, newPrec);
           if (c < best) {
             best = c;
@@ -501,9 +559,16 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
         acc.bytes_uploaded += run->bytesUp;
         acc.bytes_downloaded += run->bytesDown;
         std::vector<int> nids;
-        for (int b : live) nids.push_back(ids[b]);
+        std::vector<long long> nload;
+        for (int b : live) {
+          nids.push_back(ids[b]);
+          nload.push_back(loadIt[b]);
+        }
         ids = nids;
         B = newB;
+        nload.resize(B, it);
+        loadIt = nload;
+        limitHit.assign(B, 0);
         if (toFp64) {
           runPrec = PDHG_FP64;
           if (acc.fp64_switch_iteration < 0) acc.fp64_switch_iteration = it;
@@ -525,6 +590,12 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     for (int b = 0; b < static_cast<int>(ids.size()); ++b)
       if (!extracted[ids[b]]) rest.push_back(b);
     extract(rest, true);
+    for (int q : pending) {  // never started (time limit)
+      results[q].status = PDHG_TIME_LIMIT;
+      results[q].iterations = 0;
+      results[q].restarts = 0;
+      results[q].relative_kkt = kInf;
+    }
     if (o.kernel_timing_reps > 0 && !timed) {
       // After the results were read, every instance is released so each
       // timed launch does the full work of an iteration.

@@ -257,6 +257,58 @@ __global__ __launch_bounds__(kWG) void k_finalize(Inst st, const double* __restr
 }
 
 // ---------------------------------------------------------------------------
+// Lane loading (continuous batching)
+// ---------------------------------------------------------------------------
+// dst(i, lanes[q]) = src[i*K + q] for i < len, q < K, in the run's layout
+// (split words when `split`: FP64 at B >= 16).
+template <typename U>
+__global__ void k_scatter_lanes(U* __restrict__ dst, const U* __restrict__ src, const int* __restrict__ lanes,
+                                long long len, int K, int logB, int split) {
+  const long long total = len * K;
+  for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < total;
+       t += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const long long i = t / K;
+    const int b = lanes[t % K];
+    const U v = src[t];
+    if constexpr (sizeof(U) == 8) {
+      if (split) {
+        unsigned* u = reinterpret_cast<unsigned*>(dst) + (i << (logB + 1));
+        const unsigned long long bits = __builtin_bit_cast(unsigned long long, v);
+        u[b] = static_cast<unsigned>(bits);
+        u[(1 << logB) + b] = static_cast<unsigned>(bits >> 32);
+        continue;
+      }
+    }
+    dst[(i << logB) + b] = v;
+  }
+}
+
+// dst(i, lanes[q]) = src(i, lanes[q]) for i < len (same layout; split FP64
+// words are copied as two dwords).
+template <typename U>
+__global__ void k_copy_lanes(U* __restrict__ dst, const U* __restrict__ src, const int* __restrict__ lanes,
+                             long long len, int K, int logB, int split) {
+  const long long total = len * K;
+  for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < total;
+       t += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const long long i = t / K;
+    const int b = lanes[t % K];
+    if constexpr (sizeof(U) == 8) {
+      if (split) {
+        const size_t o = static_cast<size_t>(i) << (logB + 1);
+        unsigned* ud = reinterpret_cast<unsigned*>(dst) + o;
+        const unsigned* us = reinterpret_cast<const unsigned*>(src) + o;
+        ud[b] = us[b];
+        ud[(1 << logB) + b] = us[(1 << logB) + b];
+        continue;
+      }
+    }
+    const size_t e = (static_cast<size_t>(i) << logB) + b;
+    dst[e] = src[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Op-level kernels (the Backend vtable, reference pdhg.cpp:51-70)
 // ---------------------------------------------------------------------------
 template <typename T, int LOGB, bool SPL, int RPW = 1>
@@ -860,6 +912,97 @@ class RunImpl final : public DeviceRun {
     sel(st_.restarts, o.restarts);
   }
 
+  void load_lanes(const std::vector<int>& lanes, const double* c, const double* lo, const double* up,
+                  const double* rl, const double* ru, const double* x, const double* y, const double* omega,
+                  const double* cNorm, const double* rhsNorm) override {
+    const int K = static_cast<int>(lanes.size());
+    if (K == 0) return;
+    for (int b : lanes) PDHG_REQUIRE(b >= 0 && b < B_, "load_lanes: lane out of range");
+    const size_t maxLen = static_cast<size_t>(std::max(n_, m_));
+    if (!stageT_) {
+      stageT_ = buf_.alloc<T>(maxLen * B_);
+      stage64_ = buf_.alloc<double>(maxLen * B_);
+      laneIdx_ = buf_.alloc<int>(B_);
+      axScratch_ = buf_.alloc<T>(static_cast<size_t>(m_) * B_);
+    }
+    copy_h2d(laneIdx_, lanes.data(), K);
+    const int split = split_ ? 1 : 0;
+    auto grid = [&](size_t total) { return dim3(static_cast<unsigned>(std::min<size_t>(4096, (total + 255) / 256))); };
+    // Iteration copy (T, run layout) of a K-interleaved host array.
+    auto scatterT = [&](T* const* dsts, int ndst, const double* h, int len) {
+      std::vector<T> t(h, h + static_cast<size_t>(len) * K);
+      copy_h2d(stageT_, t.data(), t.size());
+      for (int d = 0; d < ndst; ++d)
+        hipLaunchKernelGGL(k_scatter_lanes<T>, grid(t.size()), dim3(256), 0, stream_, dsts[d], stageT_, laneIdx_,
+                           static_cast<long long>(len), K, logB_, split);
+    };
+    // FP64 copy (element order) read by the check kernels.
+    auto scatter64 = [&](double* dst, const double* h, int len) {
+      copy_h2d(stage64_, h, static_cast<size_t>(len) * K);
+      hipLaunchKernelGGL(k_scatter_lanes<double>, grid(static_cast<size_t>(len) * K), dim3(256), 0, stream_, dst,
+                         stage64_, laneIdx_, static_cast<long long>(len), K, logB_, 0);
+    };
+    T* xs[3] = {x_, x0_, xp_};
+    scatterT(xs, 3, x, n_);
+    T* ys[3] = {y_, y0_, yp_};
+    scatterT(ys, 3, y, m_);
+    if (c) {
+      scatterT(&c_, 1, c, n_);
+      scatter64(c64_, c, n_);
+    }
+    if (lo) {
+      scatterT(&lo_, 1, lo, n_);
+      scatterT(&up_, 1, up, n_);
+      scatter64(lo64_, lo, n_);
+      scatter64(up64_, up, n_);
+    }
+    if (rl) {
+      scatterT(&rl_, 1, rl, m_);
+      scatterT(&ru_, 1, ru, m_);
+      scatter64(rl64_, rl, m_);
+      scatter64(ru64_, ru, m_);
+    }
+    // ax = A x for the new lanes (all lanes computed into scratch, only the
+    // new lanes copied, so the live lanes keep their own ax).
+    launch_spmv(false, x_, axScratch_);
+    for (T* d : {ax_, ax0_, axp_})
+      hipLaunchKernelGGL(k_copy_lanes<T>, grid(static_cast<size_t>(m_) * K), dim3(256), 0, stream_, d, axScratch_,
+                         laneIdx_, static_cast<long long>(m_), K, logB_, split);
+    HIP_OK(hipGetLastError());
+    // Per-lane restart state (src/pdhg/pdhg.cpp:506-545 for a fresh case).
+    auto put_lanes = [&](auto* d, auto value_of) {
+      using U = std::remove_pointer_t<decltype(d)>;
+      std::vector<U> full(B_);
+      HIP_OK(hipMemcpyAsync(full.data(), d, B_ * sizeof(U), hipMemcpyDeviceToHost, stream_));
+      HIP_OK(hipStreamSynchronize(stream_));
+      for (int q = 0; q < K; ++q) full[lanes[q]] = static_cast<U>(value_of(q));
+      copy_h2d(d, full.data(), B_);
+    };
+    put_lanes(st_.tau, [&](int q) { return sc_.eta / omega[q]; });
+    put_lanes(st_.sigma, [&](int q) { return sc_.eta * omega[q]; });
+    put_lanes(st_.omega, [&](int q) { return omega[q]; });
+    put_lanes(st_.fpR, [&](int) { return INFINITY; });
+    put_lanes(st_.fpPrev, [&](int) { return INFINITY; });
+    put_lanes(st_.iter, [&](int) { return 0; });
+    put_lanes(st_.since, [&](int) { return 0; });
+    put_lanes(st_.restartAt, [&](int) { return 0; });
+    put_lanes(st_.applyAt, [&](int) { return -1; });
+    put_lanes(st_.doneIter, [&](int) { return -1; });
+    put_lanes(st_.restarts, [&](int) { return 0; });
+    put_lanes(cNorm_, [&](int q) { return cNorm[q]; });
+    put_lanes(rhsNorm_, [&](int q) { return rhsNorm[q]; });
+    put_lanes(st_.done, [&](int) { return 0; });
+  }
+
+  void retire_lanes(const std::vector<int>& lanes) override {
+    if (lanes.empty()) return;
+    std::vector<int> full(B_);
+    HIP_OK(hipMemcpyAsync(full.data(), st_.done, B_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    for (int b : lanes) full[b] = 1;
+    copy_h2d(st_.done, full.data(), B_);
+  }
+
   void autotune(const Prepared& P, const std::vector<MapSetup>& cand, int reps, int* best, double* ms) override {
     PDHG_REQUIRE(!cand.empty() && reps >= 1, "autotune: no candidates");
     // Scratch state: zero vectors, every lane live, no restart pending.
@@ -1078,6 +1221,9 @@ class RunImpl final : public DeviceRun {
   Kern<T> k_{};
   int m_ = 0, n_ = 0, B_ = 1, logB_ = 0, nnz_ = 0;
   bool split_ = false;
+  T *stageT_ = nullptr, *axScratch_ = nullptr;  // load_lanes staging
+  double* stage64_ = nullptr;
+  int* laneIdx_ = nullptr;
   int *aPtr_ = nullptr, *aIdx_ = nullptr, *tPtr_ = nullptr, *tIdx_ = nullptr;
   T *aVal_ = nullptr, *tVal_ = nullptr;
   double *aVal64_ = nullptr, *tVal64_ = nullptr, *rfac_ = nullptr, *cfac_ = nullptr;

@@ -326,3 +326,29 @@ def test_refinery_lp_full_size(gpu):
         assert cert["accepted"], cert
         out[prec] = r.objective
     assert abs(out[P.FP32_AUTO] - out[P.FP64]) <= 1e-4 * (1 + abs(out[P.FP64]))
+
+
+def test_streamed_family_equals_fixed_batches(gpu):
+    """More cases than one run holds: lanes freed by finished cases are
+    refilled from the queue (continuous batching).  Each case's trajectory
+    depends only on its own data, so in FP64 a case streamed through the
+    100-case run ends exactly as it does in a plain 64-case run."""
+    lp = P.gen_random_lp(11, 120, 240, 4.0, 0.0)
+    rng = np.random.default_rng(8)
+    costs = lp.c[None, :] + rng.uniform(0.0, 0.2, (100, lp.n))
+    # No compaction: every case then runs at width 64 with the same row
+    # mapping (summation order) in all three solves.
+    opts = P.default_options(tolerance=1e-7, precision=P.FP64, compact=0)
+    streamed, st, _ = P.solve_batch(lp, costs, opts)
+    head, _, _ = P.solve_batch(lp, costs[:64], opts)
+    tail, _, _ = P.solve_batch(lp, costs[36:], opts)
+    assert st["batch"] == 64
+    assert all(r.status == P.OPTIMAL for r in streamed)
+    for q in range(100):
+        ref = head[q] if q < 64 else tail[q - 36]
+        assert streamed[q].iterations == ref.iterations, q
+        assert streamed[q].objective == ref.objective, q
+        assert np.array_equal(streamed[q].x, ref.x) and np.array_equal(streamed[q].y, ref.y), q
+    # the streamed run is shorter than the two fixed batches back to back
+    print(f"streamed run {st['iterations']} iterations for 100 cases; "
+          f"max case {max(r.iterations for r in streamed)}")
