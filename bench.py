@@ -1,10 +1,12 @@
 """Benchmark of the restarted-PDHG engine on one MI355X.
 
-Workload (one "step"): the 64-instance parametric family of the refinery-style
-LP (12 periods x 40 streams x 36 products x 30 qualities, 555,852 nonzeros,
-rows/columns scaled by 10^U(-3,3); costs c_k = c0 + theta_k d on four crude
-purchase prices, theta_k ~ U[-0.3, 0.3]^4) solved as ONE batched run to
-relative KKT <= 1e-6 from a learned warm start, then every answer certified
+Workload (one "step"): 128 held-out cases of the parametric family of the
+refinery-style LP (12 periods x 40 streams x 36 products x 30 qualities,
+555,852 nonzeros, rows/columns scaled by 10^U(-3,3); costs
+c_k = c0 + theta_k d on four crude purchase prices, theta_k ~ U[-0.3, 0.3]^4)
+solved by ONE batched run of width 64 (lanes freed by finished cases take the
+next case) to relative KKT <= 1e-6 from a learned warm start, then every
+answer certified
 (implied bounds + rigorous safe dual bound, acceptance
 |obj - LB| <= 1e-4 (1 + |obj|)).  A step is: MLP inference theta -> POD
 coefficients (BF16 MFMA), BF16-MFMA decode of the starting points, upload,
@@ -39,7 +41,7 @@ import pdhg_amd as P  # noqa: E402
 import warmstart as W  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
-SEED_LP, SEED_THETA, SEED_TRAIN, K = 3, 4, 5, 64
+SEED_LP, SEED_THETA, SEED_TRAIN = 3, 4, 5
 
 
 def log(*a):
@@ -83,7 +85,10 @@ def family_step(lp, thetas, costs, opts, ws=None, threads=16):
 def cpu_baseline(lp, costs, gpu_iters, seconds, threads):
     """The reference CPU PDHG (its own sources, 16 threads) on the same batched
     family, run for a bounded number of iterations; solves/s extrapolated to the
-    iteration count the GPU batch needed."""
+    iterations its batch would run (every case iterates until the slowest one
+    stops, src/pdhg/pdhg.cpp:606-607: the largest per-case count, capped at the
+    iteration limit)."""
+    K = costs.shape[0]
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
 
@@ -104,9 +109,9 @@ def cpu_baseline(lp, costs, gpu_iters, seconds, threads):
         "unit": "solves/s",
         "cores": threads,
         "kind": "reference",
-        "sample": (f"reference batched CPU PDHG (FP64, {threads} threads) on the same 64-instance family, "
+        "sample": (f"reference batched CPU PDHG (FP64, {threads} threads) on the same {K}-case family, "
                    f"{iters} iterations in {el:.1f} s = {per * 1e3:.2f} ms/iteration, extrapolated to the "
-                   f"{gpu_iters} iterations the GPU batch needed; host nproc={os.cpu_count()}"),
+                   f"{gpu_iters} iterations its batch would run; host nproc={os.cpu_count()}"),
         "seconds_per_iteration": per,
     }
 
@@ -160,6 +165,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--precision", choices=["fp32", "fp64", "fp32auto"], default="fp64")
+    ap.add_argument("--cases", type=int, default=128, help="held-out family cases solved per step")
     ap.add_argument("--cold", action="store_true", help="no learned warm start (batched cold start only)")
     ap.add_argument("--train", type=int, default=64, help="family parameters solved to fit the warm start")
     ap.add_argument("--train-tol", type=float, default=1e-6, help="tolerance of the training solves")
@@ -205,6 +211,7 @@ def main():
         log(f"calibration copy: {args.copy_calibration} MiB device to device")
     prec = {"fp64": P.FP64, "fp32": P.FP32, "fp32auto": P.FP32_AUTO}[args.precision]
     lp = P.gen_refinery(SEED_LP)
+    K = args.cases
     thetas, costs = P.gen_family(lp, SEED_THETA, K, 0.3)
     opts = P.default_options(tolerance=args.tol, precision=prec, max_iterations=args.max_iterations,
                              progress_seconds=15.0)
@@ -236,7 +243,8 @@ def main():
             f"max rel-KKT {kkt[-1]:.2e}, max certified gap {gaps[-1]:.2e}")
     elapsed = time.perf_counter() - t0
     value = certified / elapsed
-    gpu_iters = int(max(iters))
+    gpu_iters = int(max(iters))  # iterations of the (streamed) run
+    case_iters = int(min(max(inst_iters), args.max_iterations))  # slowest case, capped
 
     # The same batched solve from a cold start (one run, for the ratio).
     cold = None
@@ -249,7 +257,7 @@ def main():
         log(f"batched cold start: {el:.2f} s, {st_c['iterations']} iterations, {ok_c.sum()}/{K} certified")
 
     # Kernel roofline: HIP events on the run's own stream, same workload shape.
-    _, kst, _ = P.solve_batch(lp, costs, P.default_options(tolerance=args.tol, precision=prec, max_iterations=64 * 8,
+    _, kst, _ = P.solve_batch(lp, costs[:64], P.default_options(tolerance=args.tol, precision=prec, max_iterations=64 * 8,
                                                           kernel_timing_reps=200))
     kern = {"k_col": (kst["col_kernel_bytes"], kst["col_kernel_ms"]),
             "k_row": (kst["row_kernel_bytes"], kst["row_kernel_ms"])}
@@ -276,13 +284,13 @@ def main():
     plain = plain_ok / (time.perf_counter() - tp) if args.plain_sample else None
     log(f"plain single-instance GPU PDHG: {plain} solves/s over {args.plain_sample} instances, iterations {plain_iters}")
 
-    cpu = None if args.no_cpu else cpu_baseline(lp, costs, gpu_iters, args.cpu_seconds, args.cpu_threads)
+    cpu = None if args.no_cpu else cpu_baseline(lp, costs, case_iters, args.cpu_seconds, args.cpu_threads)
     if cpu:
         log(f"cpu baseline: {cpu['value']:.4f} solves/s ({cpu['sample']})")
 
     mode = "batched+warm" if ws is not None else "batched"
     out = {
-        "metric": f"LP solves/s at rel-KKT<=1e-6 ({mode} 64-instance family, certified)",
+        "metric": f"LP solves/s at rel-KKT<=1e-6 ({mode} refinery family, certified)",
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
@@ -294,8 +302,9 @@ def main():
         "vs_baseline": None,
         "dtype": {P.FP64: "f64", P.FP32: "f32", P.FP32_AUTO: "f32+f64"}[prec],
         "data": "synthetic",
-        "config": {"workload": f"refinery-family-{K}: m={lp.m} n={lp.n} nnz={lp.nnz}, {K} cost cases, tol {args.tol}",
-                   "batch": K, "start": "learned warm start" if ws is not None else "cold",
+        "config": {"workload": f"refinery-family: m={lp.m} n={lp.n} nnz={lp.nnz}, {K} cost cases streamed "
+                               f"through a run of width 64, tol {args.tol}",
+                   "cases": K, "batch_width": 64, "start": "learned warm start" if ws is not None else "cold",
                    "iterations": gpu_iters, "median_instance_iterations": float(np.median(inst_iters)),
                    "certified": certified, "solved": args.steps * K},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
