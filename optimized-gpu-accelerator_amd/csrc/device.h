@@ -53,6 +53,7 @@ struct CheckRecord {
 
 struct RunScalars {
   double eta = 1, beta = 1, gamma = 1, offset = 0, tol = 1e-6;
+  long long maxIter = 1LL << 62;  // a case's own iteration limit (its last check)
 };
 
 // Host copy (FP64) of the iterate and restart state of some lanes of a run,

@@ -257,6 +257,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
   sc.gamma = P.gamma;
   sc.offset = P.offset;
   sc.tol = o.tolerance;
+  sc.maxIter = o.max_iterations;
   int runPrec = o.precision == PDHG_FP64 ? PDHG_FP64 : PDHG_FP32;
   std::vector<double> refKkt(k, kInf);
   std::vector<int> lastGain(k, 0);
@@ -409,7 +410,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
         } else {
           PDHG_REQUIRE(final || limitHit[b], "internal: extracting a live instance");
           r.status = expired ? PDHG_TIME_LIMIT : PDHG_ITERATION_LIMIT;
-          const long long own = it - loadIt[b];
+          const long long own = limitHit[b] ? o.max_iterations : it - loadIt[b];
           r.iterations = expired ? own - 1 : own;
         }
         r.restarts = restarts[b];
@@ -430,10 +431,17 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     };
 
     while (stream || it < o.max_iterations) {
-      long long nextLimit = o.max_iterations;  // fewest iterations any live lane has left
+      // The block ends at the nearest check point of a live lane, counted on
+      // the lane's own iterations: the next multiple of check_every or its
+      // iteration limit (src/pdhg/pdhg.cpp:593).  Lanes of one fixed batch
+      // share their count; streamed lanes start at block ends.
+      long long nextCheck = o.check_every;
       for (int b = 0; b < static_cast<int>(ids.size()); ++b)
-        if (!done[b]) nextLimit = std::min(nextLimit, loadIt[b] + o.max_iterations - it);
-      const int nIt = static_cast<int>(std::max<long long>(1, std::min<long long>(o.check_every, nextLimit)));
+        if (!done[b]) {
+          const long long own = it - loadIt[b];
+          nextCheck = std::min({nextCheck, o.check_every - own % o.check_every, o.max_iterations - own});
+        }
+      const int nIt = static_cast<int>(std::max<long long>(1, nextCheck));
       run->run_block(nIt);
       run->fetch(rec.data(), done.data(), doneIter.data(), restarts.data());
       it += nIt;

@@ -417,11 +417,19 @@ __device__ inline T cld(const T* __restrict__ p, const Ld& l, int i, int b) {
 // row side: primal residual^2, row dual objective, ||y - y+||^2,
 // ||y+ - y0||^2), all FP64 (reference pdhg.cpp:604-656): relative KKT,
 // convergence, the restart rule, the primal-weight update and new steps.
+// A case is checked on its own iteration count: every checkEvery iterations
+// and at its iteration limit (pdhg.cpp:593).  A block that ends elsewhere for
+// this case (cases of a continuous batch start at different run iterations)
+// only advances its count.
 __device__ inline void finalize_instance(Inst st, int b, const double (&S)[9], const RunScalars& sc, int blockIters,
                                          int checkEvery) {
   const double dx2 = S[0], ax2 = S[1], dres2 = S[2], pobjS = S[3], dobjC = S[4];
   const double pres2 = S[5], dobjR = S[6], dy2 = S[7], ay2 = S[8];
   const long long it = st.iter[b] + blockIters - 1;
+  if ((it + 1) % checkEvery != 0 && it + 1 < sc.maxIter) {
+    st.iter[b] += blockIters;
+    return;
+  }
   const double so = sc.beta * sc.gamma;
   const double pobj = pobjS * so;
   const double dobj = (dobjR + dobjC) * so;

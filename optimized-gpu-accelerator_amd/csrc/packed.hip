@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kWG) void k_pcolcheck(const PTask* __restrict__ tas
 __global__ __launch_bounds__(kWG) void k_pfinalize(Inst st, const double* __restrict__ colPart, int nColTasks,
                                                    const int* __restrict__ colBeg, const double* __restrict__ rowPart,
                                                    int nRowTasks, const int* __restrict__ rowBeg, PScal ps,
-                                                   double tol, int blockIters, int checkEvery) {
+                                                   double tol, long long maxIter, int blockIters, int checkEvery) {
   const int b = blockIdx.x;
   if (st.done[b]) return;
   double v[9];
@@ -284,6 +284,7 @@ __global__ __launch_bounds__(kWG) void k_pfinalize(Inst st, const double* __rest
   sc.gamma = ps.gamma[b];
   sc.offset = ps.offset[b];
   sc.tol = tol;
+  sc.maxIter = maxIter;
   finalize_instance(st, b, S, sc, blockIters, checkEvery);
 }
 
@@ -458,8 +459,9 @@ class PackedRun {
 
   void init_state(const std::vector<double>& omega, const std::vector<double>& cNorm,
                   const std::vector<double>& rhsNorm, const std::vector<double>& eta, const std::vector<double>& beta,
-                  const std::vector<double>& gamma, const std::vector<double>& offset, double tol) {
+                  const std::vector<double>& gamma, const std::vector<double>& offset, double tol, long long maxIter) {
     tol_ = tol;
+    maxIter_ = maxIter;
     std::vector<double> tau(K_), sigma(K_), inf(K_, INFINITY);
     for (int b = 0; b < K_; ++b) {
       tau[b] = eta[b] / omega[b];
@@ -598,7 +600,7 @@ class PackedRun {
                        queue_ + 2 * kQueueSet, colQ_, tPtr_,
                        tIdx_, tVal64_, yp_, xp_, c64_, lo64_, up64_, cfac_, st_, colPart_);
     hipLaunchKernelGGL(k_pfinalize, dim3(K_), dim3(kWG), 0, s_, st_, colPart_, nCol_, dColBeg_, rowPart_, nRow_,
-                       dRowBeg_, ps_, tol_, iters, checkEvery_);
+                       dRowBeg_, ps_, tol_, maxIter_, iters, checkEvery_);
     HIP_OK(hipGetLastError());
   }
   template <typename U>
@@ -628,6 +630,7 @@ class PackedRun {
   bool useGraph_;
   int grid_ = 8, maxGrid_ = 8, nCol_ = 0, nRow_ = 0;
   double tol_ = 1e-6;
+  long long maxIter_ = 1LL << 62;
   hipStream_t s_ = nullptr;
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
   hipGraph_t graph_ = nullptr;
@@ -771,7 +774,7 @@ PackedOutcome run_packed(const PackedData& d, const pdhg_options& o, long long m
   PackedRun<T> run(o.device, d.A, d.AT, d.rowOff, d.colOff, d.rfac, d.cfac, o.lanes_per_row > 0 ? o.lanes_per_row : 4,
                    o.check_every, o.reflection, o.use_graph != 0);
   run.upload(d.c, d.lo, d.upv, d.rl, d.ru, d.x, d.y);
-  run.init_state(d.omega, d.cNorm, d.rhsNorm, d.eta, d.beta, d.gamma, d.offset, o.tolerance);
+  run.init_state(d.omega, d.cNorm, d.rhsNorm, d.eta, d.beta, d.gamma, d.offset, o.tolerance, maxIter);
   run.start();
   PackedOutcome r;
   std::vector<double> refKkt(K, INFINITY);

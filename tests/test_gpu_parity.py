@@ -352,3 +352,36 @@ def test_streamed_family_equals_fixed_batches(gpu):
     # the streamed run is shorter than the two fixed batches back to back
     print(f"streamed run {st['iterations']} iterations for 100 cases; "
           f"max case {max(r.iterations for r in streamed)}")
+
+
+def test_streamed_family_iteration_limit(gpu):
+    """Continuous batching with an iteration limit that is not a multiple of
+    the check interval.  Each case is checked on its own iteration count,
+    every 64 iterations and at its limit (src/pdhg/pdhg.cpp:593), so a lane
+    reaching its limit mid-block does not add checks to the other lanes:
+    converged and capped cases of the streamed run equal those of plain
+    64-case runs, and a capped case reports exactly the limit."""
+    lp = P.gen_random_lp(11, 120, 240, 4.0, 0.0)
+    rng = np.random.default_rng(8)
+    costs = lp.c[None, :] + rng.uniform(0.0, 0.2, (100, lp.n))
+    opts = P.default_options(tolerance=1e-7, precision=P.FP64, compact=0)
+    free, _, _ = P.solve_batch(lp, costs[:64], opts)
+    cap = int(np.median([r.iterations for r in free])) + 37
+    cap += 1 if cap % 64 == 0 else 0
+    opts = P.default_options(tolerance=1e-7, precision=P.FP64, compact=0, max_iterations=cap)
+    streamed, st, _ = P.solve_batch(lp, costs, opts)
+    head, _, _ = P.solve_batch(lp, costs[:64], opts)
+    tail, _, _ = P.solve_batch(lp, costs[36:], opts)
+    assert st["batch"] == 64
+    status = [r.status for r in streamed]
+    assert status.count(P.ITERATION_LIMIT) >= 10 and status.count(P.OPTIMAL) >= 10, status
+    for q in range(100):
+        ref = head[q] if q < 64 else tail[q - 36]
+        r = streamed[q]
+        assert r.status == ref.status, q
+        assert r.iterations == ref.iterations, q
+        assert r.iterations <= cap, q
+        if r.status == P.ITERATION_LIMIT:
+            assert r.iterations == cap, q
+        assert r.objective == ref.objective and r.relative_kkt == ref.relative_kkt, q
+        assert np.array_equal(r.x, ref.x) and np.array_equal(r.y, ref.y), q
