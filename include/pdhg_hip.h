@@ -99,7 +99,8 @@ typedef struct pdhg_options {
   int precision;               /* PDHG_FP32 (default), PDHG_FP64 or PDHG_FP32_AUTO */
   int use_graph;               /* capture check_every iterations in a hipGraph (1) */
   int lanes_per_row;           /* 0 = autotune; else nnz-per-lane target (1..64) */
-  int row_map;                 /* 0 = auto, 1 = row-length buckets, 2 = one mapping */
+  int row_map;                 /* 0 = auto, 1 = row-length buckets, 2 = one mapping,
+                                  3 = LDS-staged row tiles (width 64; else as 1) */
   int xcd_remap;               /* -1 = auto, 0 = off, 1 = contiguous rows per XCD */
   double time_limit;           /* seconds, <= 0: none (Deadline, src/util/timer.h:22) */
   int device;                  /* HIP device ordinal (0) */

@@ -33,6 +33,9 @@ struct MapSetup {
   bool group = false;
   // B = 64: rows per wavefront (1, or 2 with their chunks interleaved).
   int rowsPerWave = 1;
+  // B = 64, iterations without checks: rows in locality order, grouped in
+  // tiles whose distinct gathered rows a workgroup stages in LDS once.
+  bool tile = false;
 };
 
 struct RunSetup {
@@ -117,6 +120,9 @@ class DeviceRun {
   // chosen candidate index for the column / row kernel, ms[0] / ms[1] its
   // mean launch time.
   virtual void autotune(const Prepared& P, const std::vector<MapSetup>& cand, int reps, int* best, double* ms) = 0;
+  // Whether the iterations without checks run in LDS-staged tiles
+  // (MapSetup::tile), for the column (0) or row (1) kernel.
+  virtual bool tiled(int side) const = 0;
 
   long long launches = 0, graphLaunches = 0, bytesUp = 0, bytesDown = 0;
 };

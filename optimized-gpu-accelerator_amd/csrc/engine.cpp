@@ -132,6 +132,7 @@ std::vector<MapSetup> map_candidates(int B) {
   for (bool xcd : {false, true}) {
     if (B == 64) {
       for (int rpw : {1, 2}) c.push_back(MapSetup{4, 1, xcd, false, rpw});
+      c.push_back(MapSetup{4, 1, xcd, false, 1, true});  // rows in LDS-staged tiles
       continue;
     }
     for (int per : {2, 4, 8, 16})
@@ -277,6 +278,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     MapSetup ms;
     ms.nnzPerLane = o.lanes_per_row > 0 ? o.lanes_per_row : 4;
     ms.rowMap = o.row_map == 2 ? 2 : 1;
+    ms.tile = o.row_map == 3 && B == 64;
     ms.xcd = o.xcd_remap == 1 || (o.xcd_remap < 0 && B >= 16);
     return ms;
   };
@@ -332,6 +334,9 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       tunedFirst = autoMap;
       firstCol = rs.colMap;
       firstRow = rs.rowMap;
+      // Tiles need rows short enough for the staging LDS; else the plain plan.
+      firstCol.tile = run->tiled(0);
+      firstRow.tile = run->tiled(1);
     }
     LaneData d = lane_data(ids, B);
     run->upload(d.c.data(), d.lo.data(), d.up.data(), d.rl.data(), d.ru.data(), d.x.data(), d.y.data());
@@ -637,10 +642,10 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     std::snprintf(acc.device_name, sizeof(acc.device_name), "%s", di.name.c_str());
     acc.autotuned = tunedFirst ? 1 : 0;
     acc.col_nnz_per_lane = firstCol.group ? 0 : firstCol.nnzPerLane;
-    acc.col_row_map = firstCol.rowMap;
+    acc.col_row_map = firstCol.tile ? 3 : firstCol.rowMap;
     acc.col_xcd = firstCol.xcd ? 1 : 0;
     acc.row_nnz_per_lane = firstRow.group ? 0 : firstRow.nnzPerLane;
-    acc.row_row_map = firstRow.rowMap;
+    acc.row_row_map = firstRow.tile ? 3 : firstRow.rowMap;
     acc.row_xcd = firstRow.xcd ? 1 : 0;
     acc.tune_seconds = tuneSeconds;
     acc.col_rows_per_wave = firstCol.rowsPerWave;

@@ -44,7 +44,22 @@ struct RowPlan {
   int xcd;                         // 1: contiguous row ranges per XCD
   int grp;                         // 1: one row per B-lane group, entries via the scalar cache
   int rpw;                         // B = 64: rows per wavefront (1 or 2)
+  // B = 64 tiles (one workgroup each): rows perm[tileRow[t] .. tileRow[t+1]),
+  // whose distinct columns ucols[tileCol[t] .. tileCol[t+1]) are staged in LDS.
+  const int* tileRow;
+  const int* tileCol;
+  const int* ucols;
 };
+
+// Tiles: a 1024-thread workgroup (16 wavefronts, one row each) per tile and
+// the LDS it stages the tile's columns in; two such workgroups fill a CU's 32
+// wavefront slots.
+constexpr int kTileWaves = 16;
+constexpr int kTileWG = 64 * kTileWaves;
+constexpr int kTileRows = kTileWaves;
+constexpr int kTileBytes = 48 * 1024;
+template <typename T>
+constexpr int kTileCols = kTileBytes / (64 * static_cast<int>(sizeof(T)));
 
 // Workgroup ids are dispatched round-robin over the 8 XCDs; this bijection
 // gives XCD x a contiguous range of logical blocks, so neighbouring rows (which
@@ -181,6 +196,79 @@ __device__ inline Acc row_dot_wave(const int* __restrict__ idx, const V* __restr
     for (int u = 0; u < CH; ++u) a[u & 3] += v[u] * g[u];
   }
   return (a[0] + a[1]) + (a[2] + a[3]);
+}
+
+// B = 64 over a staged tile: row_dot_wave with the gathers read from LDS,
+// where tile column u of lane b sits at tile[u * 64 + b], and loc[e] the tile
+// column of entry e.  Same chunks, predication and summation order.
+template <typename Acc, typename V, typename X>
+__device__ inline Acc row_dot_lds(const int* __restrict__ loc, const V* __restrict__ val, const X* tile, int e0,
+                                  int e1, int b) {
+  constexpr int CH = 8;
+  e0 = __builtin_amdgcn_readfirstlane(e0);
+  e1 = __builtin_amdgcn_readfirstlane(e1);
+  Acc a[4] = {0, 0, 0, 0};
+  int e = e0;
+  for (; e + CH <= e1; e += CH) {
+    int c[CH];
+    Acc v[CH], g[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      c[u] = loc[e + u];
+      v[u] = static_cast<Acc>(val[e + u]);
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) g[u] = static_cast<Acc>(tile[c[u] * 64 + b]);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) a[u & 3] += v[u] * g[u];
+  }
+  if (e < e1) {
+    const int last = e1 - 1;
+    int c[CH];
+    Acc v[CH], g[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int p = min(e + u, last);
+      c[u] = loc[p];
+      const Acc w = static_cast<Acc>(val[p]);
+      v[u] = e + u <= last ? w : static_cast<Acc>(0);
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) g[u] = static_cast<Acc>(tile[c[u] * 64 + b]);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) a[u & 3] += v[u] * g[u];
+  }
+  return (a[0] + a[1]) + (a[2] + a[3]);
+}
+
+// Staging of tile t of a B = 64 plan in LDS: column u - u0 of lane b holds
+// src row ucols[u] of lane b (src may differ per lane).  tile_load issues a
+// wavefront's share of the loads, tile_store writes them to LDS; a kernel
+// issues its other loads in between, then meets the others at a barrier.
+template <typename T>
+constexpr int kStageCols = (kTileCols<T> + kTileWaves - 1) / kTileWaves;  // per wavefront at most
+
+template <bool SPLIT, typename T>
+__device__ inline void tile_load(const RowPlan& p, int t, const T* src, int b, T (&v)[kStageCols<T>]) {
+  const int u0 = p.tileCol[t], u1 = p.tileCol[t + 1];
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+#pragma unroll
+  for (int k = 0; k < kStageCols<T>; ++k) {
+    const int uu = u0 + wave + k * kTileWaves;
+    if (uu < u1) v[k] = vld<SPLIT, 6>(src, p.ucols[uu], b);
+  }
+}
+
+template <typename T>
+__device__ inline void tile_store(const RowPlan& p, int t, const T (&v)[kStageCols<T>], T* tile, int b) {
+  const int u0 = p.tileCol[t], u1 = p.tileCol[t + 1];
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+#pragma unroll
+  for (int k = 0; k < kStageCols<T>; ++k) {
+    const int uu = u0 + wave + k * kTileWaves;
+    if (uu < u1) tile[(uu - u0) * 64 + b] = v[k];
+  }
+  __syncthreads();
 }
 
 // Zero entries (column 0, weight 0) after the last nonzero of the CSR arrays

@@ -44,6 +44,58 @@ namespace {
 // 116-125, 138-152): x+ = clamp(x - tau (c - A^T y), lo, up), then
 // x <- lam((1+rho) x+ - rho x) + (1-lam) x0 with lam = (k+1)/(k+2).
 // CHECK adds ||x - x+||^2 and ||x+ - x0||^2 (quantities 0, 1).
+// col_update is that step for column j of lane b, given acc = (A^T y)_j;
+// col_load / col_apply split it around the dot product (operands first).
+template <typename T>
+struct ColOps {
+  T xin, xo, c, lo, up;
+};
+
+template <typename T, int LOGB>
+__device__ inline ColOps<T> col_load(int j, int b, bool rst, const T* __restrict__ x, const T* __restrict__ xp,
+                                     const T* __restrict__ x0, const T* __restrict__ c, const Ld& lc,
+                                     const T* __restrict__ lo, const T* __restrict__ up, const Ld& lb) {
+  constexpr bool SPL = kSplit<T, LOGB>;
+  ColOps<T> o;
+  o.xin = vld<SPL, LOGB>(rst ? xp : x, j, b);
+  o.xo = rst ? o.xin : vld<SPL, LOGB>(x0, j, b);
+  o.c = cld<SPL, LOGB>(c, lc, j, b);
+  o.lo = cld<SPL, LOGB>(lo, lb, j, b);
+  o.up = cld<SPL, LOGB>(up, lb, j, b);
+  return o;
+}
+
+template <typename T, int LOGB, bool CHECK>
+__device__ inline void col_apply(int j, T acc, const ColOps<T>& o, int b, bool rst, long long it, T* __restrict__ x,
+                                 T* __restrict__ xp, T* __restrict__ x0, const Inst& st, double rho,
+                                 double (&q)[2]) {
+  constexpr bool SPL = kSplit<T, LOGB>;
+  if (rst) vst<SPL, LOGB>(x0, j, b, o.xin);
+  const T tau = static_cast<T>(st.tau[b]);
+  const T v = clamp_to<T>(o.xin - tau * (o.c - acc), o.lo, o.up);
+  const double kk = static_cast<double>(it - st.restartAt[b]);
+  const double lam = (kk + 1.0) / (kk + 2.0);
+  vst<SPL, LOGB>(x, j, b,
+                 static_cast<T>(lam) * (static_cast<T>(1.0 + rho) * v - static_cast<T>(rho) * o.xin) +
+                     static_cast<T>(1.0 - lam) * o.xo);
+  vst<SPL, LOGB>(xp, j, b, v);
+  if (CHECK) {
+    const double d = static_cast<double>(o.xin) - static_cast<double>(v);
+    const double e = static_cast<double>(v) - static_cast<double>(o.xo);
+    q[0] += d * d;
+    q[1] += e * e;
+  }
+}
+
+template <typename T, int LOGB, bool CHECK>
+__device__ inline void col_update(int j, T acc, int b, bool rst, long long it, T* __restrict__ x,
+                                  T* __restrict__ xp, T* __restrict__ x0, const T* __restrict__ c, const Ld& lc,
+                                  const T* __restrict__ lo, const T* __restrict__ up, const Ld& lb, const Inst& st,
+                                  double rho, double (&q)[2]) {
+  const ColOps<T> o = col_load<T, LOGB>(j, b, rst, x, xp, x0, c, lc, lo, up, lb);
+  col_apply<T, LOGB, CHECK>(j, acc, o, b, rst, it, x, xp, x0, st, rho, q);
+}
+
 template <typename T, int LOGB, bool CHECK, int RPW = 1>
 __global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict__ ptr, const int* __restrict__ idx,
                                              const T* __restrict__ val, const T* __restrict__ y,
@@ -67,31 +119,7 @@ __global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict
   double q[2] = {0.0, 0.0};
   for (int rr = 0; rr < 2; ++rr) {
     if (rows[rr] < 0 || s.sub != 0 || st.done[b]) continue;
-    const int j = rows[rr];
-    const T acc = accs[rr];
-    const T xin = vld<SPL, LOGB>(rst ? xp : x, j, b);
-    T xo;
-    if (rst) {
-      xo = xin;
-      vst<SPL, LOGB>(x0, j, b, xin);
-    } else {
-      xo = vld<SPL, LOGB>(x0, j, b);
-    }
-    const T tau = static_cast<T>(st.tau[b]);
-    const T v = clamp_to<T>(xin - tau * (cld<SPL, LOGB>(c, lc, j, b) - acc), cld<SPL, LOGB>(lo, lb, j, b),
-                            cld<SPL, LOGB>(up, lb, j, b));
-    const double kk = static_cast<double>(it - st.restartAt[b]);
-    const double lam = (kk + 1.0) / (kk + 2.0);
-    vst<SPL, LOGB>(x, j, b,
-                   static_cast<T>(lam) * (static_cast<T>(1.0 + rho) * v - static_cast<T>(rho) * xin) +
-                       static_cast<T>(1.0 - lam) * xo);
-    vst<SPL, LOGB>(xp, j, b, v);
-    if (CHECK) {
-      const double d = static_cast<double>(xin) - static_cast<double>(v);
-      const double e = static_cast<double>(v) - static_cast<double>(xo);
-      q[0] += d * d;
-      q[1] += e * e;
-    }
+    col_update<T, LOGB, CHECK>(rows[rr], accs[rr], b, rst, it, x, xp, x0, c, lc, lo, up, lb, st, rho, q);
   }
   if (CHECK) block_partials<LOGB, 2>(q, part, 0, nblk, bid);
 }
@@ -102,6 +130,83 @@ __global__ __launch_bounds__(kWG) void k_col(RowPlan plan, const int* __restrict
 // CHECK also stores y+ and ax+, recomputes A x+ with FP64 values and FP64
 // accumulation, and adds (quantities 0..3): sum res^2, the row part of the
 // dual objective, ||y - y+||^2, ||y+ - y0||^2.
+// row_update is that step for row i of lane b, given acc = (A x+)_i (and its
+// FP64 recomputation acc64 when CHECK); row_load / row_apply split it around
+// the dot product (operands first).
+template <typename T>
+struct RowOps {
+  T yin, axin, yo, axo, rl, ru;
+};
+
+template <typename T, int LOGB>
+__device__ inline RowOps<T> row_load(int i, int b, bool rst, const T* __restrict__ y, const T* __restrict__ yp,
+                                     const T* __restrict__ y0, const T* __restrict__ ax, const T* __restrict__ axp,
+                                     const T* __restrict__ ax0, const T* __restrict__ rl, const T* __restrict__ ru,
+                                     const Ld& lr) {
+  constexpr bool SPL = kSplit<T, LOGB>;
+  RowOps<T> o;
+  o.yin = vld<SPL, LOGB>(rst ? yp : y, i, b);
+  o.axin = vld<SPL, LOGB>(rst ? axp : ax, i, b);
+  o.yo = rst ? o.yin : vld<SPL, LOGB>(y0, i, b);
+  o.axo = rst ? o.axin : vld<SPL, LOGB>(ax0, i, b);
+  o.rl = cld<SPL, LOGB>(rl, lr, i, b);
+  o.ru = cld<SPL, LOGB>(ru, lr, i, b);
+  return o;
+}
+
+template <typename T, int LOGB, bool CHECK>
+__device__ inline void row_apply(int i, T acc, double acc64, const RowOps<T>& o, int b, bool rst, long long it,
+                                 T* __restrict__ y, T* __restrict__ yp, T* __restrict__ y0, T* __restrict__ ax,
+                                 T* __restrict__ axp, T* __restrict__ ax0, const double* __restrict__ rl64,
+                                 const double* __restrict__ ru64, const Ld& lr, const double* __restrict__ rfac,
+                                 const Inst& st, double rho, double (&q)[4]) {
+  constexpr bool SPL = kSplit<T, LOGB>;
+  if (rst) {
+    vst<SPL, LOGB>(y0, i, b, o.yin);
+    vst<SPL, LOGB>(ax0, i, b, o.axin);
+  }
+  // Reflected dual step y+ = v + sigma clamp(-v/sigma, L, U), v = y - sigma g,
+  // g = 2 ax+ - ax (reference pdhg.cpp:126-137), evaluated as
+  // y+ = y - sigma (g - w) with g - w in {g - L, g - U, y / sigma}: the
+  // large activity g only meets the bound it is compared with, so y keeps
+  // its low-order bits in FP32 (v + sigma w would cancel them away).
+  const T sg = static_cast<T>(st.sigma[b]);
+  const T g = static_cast<T>(2) * acc - o.axin;
+  const T tq = o.yin / sg;
+  const T dlo = g - o.rl, dhi = g - o.ru;
+  const T yplus = dlo < tq ? o.yin - sg * dlo : (dhi > tq ? o.yin - sg * dhi : static_cast<T>(0));
+  const double kk = static_cast<double>(it - st.restartAt[b]);
+  const double lam = (kk + 1.0) / (kk + 2.0);
+  const T lamT = static_cast<T>(lam), omT = static_cast<T>(1.0 - lam);
+  const T r1 = static_cast<T>(1.0 + rho), r0 = static_cast<T>(rho);
+  vst<SPL, LOGB>(y, i, b, lamT * (r1 * yplus - r0 * o.yin) + omT * o.yo);
+  vst<SPL, LOGB>(ax, i, b, lamT * (r1 * acc - r0 * o.axin) + omT * o.axo);
+  if (CHECK) {
+    vst<SPL, LOGB>(yp, i, b, yplus);
+    vst<SPL, LOGB>(axp, i, b, acc);
+    const double L = rl64[lr.at(i, b)], U = ru64[lr.at(i, b)];
+    const double res = (fmax(L - acc64, 0.0) + fmax(acc64 - U, 0.0)) * rfac[i];
+    const double yy = static_cast<double>(yplus);
+    q[0] += res * res;
+    if (yy > 0 && L > -1e300) q[1] += yy * L;
+    else if (yy < 0 && U < 1e300) q[1] += yy * U;
+    const double d = static_cast<double>(o.yin) - yy, e = yy - static_cast<double>(o.yo);
+    q[2] += d * d;
+    q[3] += e * e;
+  }
+}
+
+template <typename T, int LOGB, bool CHECK>
+__device__ inline void row_update(int i, T acc, double acc64, int b, long long it, T* __restrict__ y,
+                                  T* __restrict__ yp, T* __restrict__ y0, T* __restrict__ ax, T* __restrict__ axp,
+                                  T* __restrict__ ax0, const T* __restrict__ rl, const T* __restrict__ ru,
+                                  const double* __restrict__ rl64, const double* __restrict__ ru64, const Ld& lr,
+                                  const double* __restrict__ rfac, const Inst& st, double rho, double (&q)[4]) {
+  const bool rst = st.applyAt[b] == it;
+  const RowOps<T> o = row_load<T, LOGB>(i, b, rst, y, yp, y0, ax, axp, ax0, rl, ru, lr);
+  row_apply<T, LOGB, CHECK>(i, acc, acc64, o, b, rst, it, y, yp, y0, ax, axp, ax0, rl64, ru64, lr, rfac, st, rho, q);
+}
+
 template <typename T, int LOGB, bool CHECK, int RPW = 1>
 __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict__ ptr, const int* __restrict__ idx,
                                              const T* __restrict__ val, const double* __restrict__ val64,
@@ -122,56 +227,100 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
   lane_rows<LOGB, RPW, T, SPL>(plan, bid, s, ptr, idx, val, xp, b, rows, accs);
   if (CHECK) lane_rows<LOGB, RPW, double, SPL>(plan, bid, s, ptr, idx, val64, xp, b, rows, accs64);
   double q[4] = {0.0, 0.0, 0.0, 0.0};
+  const long long it = st.iter[b] + tloc;
   for (int rr = 0; rr < 2; ++rr) {
     if (rows[rr] < 0 || s.sub != 0 || st.done[b]) continue;
-    const int i = rows[rr];
-    const T acc = accs[rr];
-    const double acc64 = accs64[rr];
-    const long long it = st.iter[b] + tloc;
-    const bool rst = st.applyAt[b] == it;
-    const T yin = vld<SPL, LOGB>(rst ? yp : y, i, b);
-    const T axin = vld<SPL, LOGB>(rst ? axp : ax, i, b);
-    T yo, axo;
-    if (rst) {
-      yo = yin;
-      axo = axin;
-      vst<SPL, LOGB>(y0, i, b, yin);
-      vst<SPL, LOGB>(ax0, i, b, axin);
-    } else {
-      yo = vld<SPL, LOGB>(y0, i, b);
-      axo = vld<SPL, LOGB>(ax0, i, b);
-    }
-    // Reflected dual step y+ = v + sigma clamp(-v/sigma, L, U), v = y - sigma g,
-    // g = 2 ax+ - ax (reference pdhg.cpp:126-137), evaluated as
-    // y+ = y - sigma (g - w) with g - w in {g - L, g - U, y / sigma}: the
-    // large activity g only meets the bound it is compared with, so y keeps
-    // its low-order bits in FP32 (v + sigma w would cancel them away).
-    const T sg = static_cast<T>(st.sigma[b]);
-    const T g = static_cast<T>(2) * acc - axin;
-    const T tq = yin / sg;
-    const T dlo = g - cld<SPL, LOGB>(rl, lr, i, b), dhi = g - cld<SPL, LOGB>(ru, lr, i, b);
-    const T yplus = dlo < tq ? yin - sg * dlo : (dhi > tq ? yin - sg * dhi : static_cast<T>(0));
-    const double kk = static_cast<double>(it - st.restartAt[b]);
-    const double lam = (kk + 1.0) / (kk + 2.0);
-    const T lamT = static_cast<T>(lam), omT = static_cast<T>(1.0 - lam);
-    const T r1 = static_cast<T>(1.0 + rho), r0 = static_cast<T>(rho);
-    vst<SPL, LOGB>(y, i, b, lamT * (r1 * yplus - r0 * yin) + omT * yo);
-    vst<SPL, LOGB>(ax, i, b, lamT * (r1 * acc - r0 * axin) + omT * axo);
-    if (CHECK) {
-      vst<SPL, LOGB>(yp, i, b, yplus);
-      vst<SPL, LOGB>(axp, i, b, acc);
-      const double L = rl64[lr.at(i, b)], U = ru64[lr.at(i, b)];
-      const double res = (fmax(L - acc64, 0.0) + fmax(acc64 - U, 0.0)) * rfac[i];
-      const double yy = static_cast<double>(yplus);
-      q[0] += res * res;
-      if (yy > 0 && L > -1e300) q[1] += yy * L;
-      else if (yy < 0 && U < 1e300) q[1] += yy * U;
-      const double d = static_cast<double>(yin) - yy, e = yy - static_cast<double>(yo);
-      q[2] += d * d;
-      q[3] += e * e;
-    }
+    row_update<T, LOGB, CHECK>(rows[rr], accs[rr], accs64[rr], b, it, y, yp, y0, ax, axp, ax0, rl, ru, rl64, ru64,
+                               lr, rfac, st, rho, q);
   }
   if (CHECK) block_partials<LOGB, 4>(q, part, 0, nblk, bid);
+}
+
+// B = 64 tiled forms of k_col / k_row (no check quantities): a workgroup
+// takes one tile (kTileRows rows in locality order), each wavefront every
+// kTileWaves-th row of it.  One round of loads brings the tile's distinct
+// gathered rows (staged in LDS) and the wavefront's update operands; after
+// one barrier the dot products read LDS.  idx holds tile-local columns.
+template <typename T>
+__global__ __launch_bounds__(kTileWG) void k_col_tiled(RowPlan plan, const int* __restrict__ ptr,
+                                                   const int* __restrict__ loc, const T* __restrict__ val,
+                                                   const T* __restrict__ y, const T* __restrict__ yp,
+                                                   T* __restrict__ x, T* __restrict__ xp, T* __restrict__ x0,
+                                                   const T* __restrict__ c, Ld lc, const T* __restrict__ lo,
+                                                   const T* __restrict__ up, Ld lb, Inst st, int tloc, double rho,
+                                                   double* __restrict__, int) {
+  constexpr bool SPL = kSplit<T, 6>;
+  constexpr int RW = (kTileRows + kTileWaves - 1) / kTileWaves;
+  __shared__ T tile[kTileCols<T> * 64];
+  const int t = block_id(plan);
+  const int b = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const long long it = st.iter[b] + tloc;
+  const bool rst = st.applyAt[b] == it;
+  T stage[kStageCols<T>];
+  tile_load<SPL>(plan, t, rst ? yp : y, b, stage);
+  const int r0 = plan.tileRow[t] + wave, r1 = plan.tileRow[t + 1];
+  int rows[RW];
+  ColOps<T> ops[RW];
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    const int r = r0 + k * kTileWaves;
+    rows[k] = r < r1 ? plan.perm[r] : -1;
+    if (rows[k] >= 0) ops[k] = col_load<T, 6>(rows[k], b, rst, x, xp, x0, c, lc, lo, up, lb);
+  }
+  tile_store(plan, t, stage, tile, b);
+  const bool live = !st.done[b];
+  double q[2] = {0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    if (rows[k] < 0) continue;
+    const int j = rows[k];
+    const T acc = row_dot_lds<T>(loc, val, tile, ptr[j], ptr[j + 1], b);
+    if (live) col_apply<T, 6, false>(j, acc, ops[k], b, rst, it, x, xp, x0, st, rho, q);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTileWG) void k_row_tiled(RowPlan plan, const int* __restrict__ ptr,
+                                                   const int* __restrict__ loc, const T* __restrict__ val,
+                                                   const double* __restrict__, const T* __restrict__ xp,
+                                                   T* __restrict__ y, T* __restrict__ yp, T* __restrict__ y0,
+                                                   T* __restrict__ ax, T* __restrict__ axp, T* __restrict__ ax0,
+                                                   const T* __restrict__ rl, const T* __restrict__ ru,
+                                                   const double* __restrict__ rl64, const double* __restrict__ ru64,
+                                                   Ld lr, const double* __restrict__ rfac, Inst st, int tloc,
+                                                   double rho, double* __restrict__, int) {
+  constexpr bool SPL = kSplit<T, 6>;
+  constexpr int RW = (kTileRows + kTileWaves - 1) / kTileWaves;
+  __shared__ T tile[kTileCols<T> * 64];
+  const int t = block_id(plan);
+  const int b = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const long long it = st.iter[b] + tloc;
+  const bool rst = st.applyAt[b] == it;
+  T stage[kStageCols<T>];
+  tile_load<SPL>(plan, t, xp, b, stage);
+  const int r0 = plan.tileRow[t] + wave, r1 = plan.tileRow[t + 1];
+  int rows[RW];
+  RowOps<T> ops[RW];
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    const int r = r0 + k * kTileWaves;
+    rows[k] = r < r1 ? plan.perm[r] : -1;
+    if (rows[k] >= 0) ops[k] = row_load<T, 6>(rows[k], b, rst, y, yp, y0, ax, axp, ax0, rl, ru, lr);
+  }
+  tile_store(plan, t, stage, tile, b);
+  const bool live = !st.done[b];
+  double q[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    if (rows[k] < 0) continue;
+    const int i = rows[k];
+    const T acc = row_dot_lds<T>(loc, val, tile, ptr[i], ptr[i + 1], b);
+    if (live)
+      row_apply<T, 6, false>(i, acc, 0.0, ops[k], b, rst, it, y, yp, y0, ax, axp, ax0, rl64, ru64, lr, rfac, st,
+                             rho, q);
+  }
 }
 
 // Column KKT terms at (x+, y+) in FP64 (reference pdhg.cpp:181-196):
@@ -453,9 +602,71 @@ struct PlanHost {
   std::vector<int> perm;
   int* dPerm = nullptr;
   int blocks = 0;
+  // Tiles for the iterations without checks (B = 64, MapSetup::tile); the
+  // check kernels keep `plan`.
+  bool tiled = false;
+  RowPlan tplan{};
+  std::vector<int> tperm, tileRow, tileCol, ucols, loc;
+  int* dLoc = nullptr;
+  int tblocks = 0;
 };
 
-PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms) {
+// Tiles of at most kTileRows rows and maxCols distinct columns over the rows
+// in locality order (by the column of their middle entry), so that rows of a
+// tile share columns; loc[e] is entry e's column numbered within its tile.
+// False when a row alone has more than maxCols entries.
+bool build_tiles(const Csr& M, int maxCols, PlanHost& h) {
+  const int rows = M.rows;
+  for (int i = 0; i < rows; ++i)
+    if (M.ptr[i + 1] - M.ptr[i] > maxCols) return false;
+  std::vector<int> key(rows);
+  for (int i = 0; i < rows; ++i) {
+    const int len = M.ptr[i + 1] - M.ptr[i];
+    key[i] = len ? M.idx[M.ptr[i] + len / 2] : 0;
+  }
+  h.tperm.resize(rows);
+  for (int i = 0; i < rows; ++i) h.tperm[i] = i;
+  std::stable_sort(h.tperm.begin(), h.tperm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  h.loc.assign(static_cast<size_t>(M.nnz()) + kCsrPad, 0);
+  h.tileRow.assign(1, 0);
+  h.tileCol.assign(1, 0);
+  h.ucols.clear();
+  std::vector<int> mark(M.cols, -1), pos(M.cols, 0);
+  int tile = 0, inTile = 0;
+  for (int r = 0; r < rows; ++r) {
+    const int i = h.tperm[r];
+    for (int pass = 0; pass < 2; ++pass) {
+      const size_t keep = h.ucols.size();
+      bool fits = inTile < kTileRows;
+      for (int e = M.ptr[i]; fits && e < M.ptr[i + 1]; ++e) {
+        const int c = M.idx[e];
+        if (mark[c] == tile) continue;
+        mark[c] = tile;
+        pos[c] = static_cast<int>(h.ucols.size()) - h.tileCol.back();
+        h.ucols.push_back(c);
+        fits = pos[c] < maxCols;
+      }
+      if (fits) break;
+      // Close the tile (dropping this row's new columns) and retry in a new one.
+      for (size_t u = keep; u < h.ucols.size(); ++u) mark[h.ucols[u]] = -1;
+      h.ucols.resize(keep);
+      h.tileRow.push_back(r);
+      h.tileCol.push_back(static_cast<int>(h.ucols.size()));
+      ++tile;
+      inTile = 0;
+    }
+    for (int e = M.ptr[i]; e < M.ptr[i + 1]; ++e) h.loc[e] = pos[M.idx[e]];
+    ++inTile;
+  }
+  if (inTile > 0 || rows == 0) {
+    h.tileRow.push_back(rows);
+    h.tileCol.push_back(static_cast<int>(h.ucols.size()));
+  }
+  h.tblocks = static_cast<int>(h.tileRow.size()) - 1;
+  return h.tblocks > 0;
+}
+
+PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms, int tileCols = 0) {
   PlanHost h;
   const int rows = M.rows;
   const int maxLogS = 6 - logB;  // B * S <= 64 lanes (one wavefront per row at most)
@@ -530,6 +741,11 @@ PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms) {
   h.blocks = std::max(blk, 1);
   h.plan.xcd = ms.xcd ? 1 : 0;
   h.plan.perm = nullptr;
+  if (ms.tile && logB == 6 && tileCols > 0 && build_tiles(M, tileCols, h)) {
+    h.tiled = true;
+    h.tplan = h.plan;
+    h.tplan.xcd = ms.xcd ? 1 : 0;
+  }
   return h;
 }
 
@@ -599,6 +815,8 @@ struct Kern {
   ColCheckFn colcheck[2];
   SpmvFn spmv[2];  // the run's vector layout
   SpmvFn spmvStd;  // element (i, b) at i*B + b (op-level backend)
+  ColFn colTiled = nullptr;  // B = 64 tiles
+  RowFn rowTiled = nullptr;
 };
 
 template <typename T, int LOGB>
@@ -618,6 +836,10 @@ Kern<T> kern_for() {
   k.colcheck[1] = k_colcheck<T, LOGB, R2>;
   k.spmv[1] = k_spmv<T, LOGB, kSplit<T, LOGB>, R2>;
   k.spmvStd = k_spmv<T, LOGB, false>;
+  if constexpr (LOGB == 6) {
+    k.colTiled = k_col_tiled<T>;
+    k.rowTiled = k_row_tiled<T>;
+  }
   return k;
 }
 
@@ -667,12 +889,9 @@ class RunImpl final : public DeviceRun {
     }
     rfac_ = upload_dbl(P.rfac);
     cfac_ = upload_dbl(P.cfac);
-    colPlan_ = build_plan(P.AT, logB_, s.colMap);
-    rowPlan_ = build_plan(P.A, logB_, s.rowMap);
-    for (PlanHost* h : {&colPlan_, &rowPlan_}) {
-      if (!h->perm.empty()) h->dPerm = upload_int(h->perm);
-      h->plan.perm = h->dPerm;
-    }
+    colPlan_ = build_plan(P.AT, logB_, s.colMap, kTileCols<T>);
+    rowPlan_ = build_plan(P.A, logB_, s.rowMap, kTileCols<T>);
+    for (PlanHost* h : {&colPlan_, &rowPlan_}) upload_plan(*h);
     // Iterates (per instance) and case data (per instance or shared).
     const size_t nB = static_cast<size_t>(n_) * B_, mB = static_cast<size_t>(m_) * B_;
     lc_ = s.sharedCost ? Ld{1, 0} : Ld{B_, B_ - 1};
@@ -1028,9 +1247,13 @@ class RunImpl final : public DeviceRun {
       double bestMs = 0;
       int bestI = -1;
       for (size_t i = 0; i < cand.size(); ++i) {
-        PlanHost h = build_plan(M, logB_, cand[i]);
-        if (!h.perm.empty()) copy_h2d(scratch, h.perm.data(), h.perm.size());
-        h.plan.perm = h.perm.empty() ? nullptr : scratch;
+        PlanHost h = build_plan(M, logB_, cand[i], kTileCols<T>);
+        if (h.tiled) {
+          upload_plan(h);  // tiles: their own arrays, kept with the run's buffers
+        } else {
+          if (!h.perm.empty()) copy_h2d(scratch, h.perm.data(), h.perm.size());
+          h.plan.perm = h.perm.empty() ? nullptr : scratch;
+        }
         const PlanHost keep = cur;
         cur = h;
         auto one = [&] { dir == 0 ? launch_col(false, 0) : launch_row(false, 0); };
@@ -1050,9 +1273,8 @@ class RunImpl final : public DeviceRun {
         }
       }
       // Install the winner with its own permutation and partial buffers.
-      PlanHost h = build_plan(M, logB_, cand[bestI]);
-      if (!h.perm.empty()) h.dPerm = upload_int(h.perm);
-      h.plan.perm = h.dPerm;
+      PlanHost h = build_plan(M, logB_, cand[bestI], kTileCols<T>);
+      upload_plan(h);
       cur = h;
       best[dir] = bestI;
       ms[dir] = bestMs;
@@ -1061,6 +1283,8 @@ class RunImpl final : public DeviceRun {
     rowPart_ = buf_.alloc<double>(static_cast<size_t>(4) * B_ * rowPlan_.blocks);
     launches = 0;
   }
+
+  bool tiled(int side) const override { return side == 0 ? colPlan_.tiled : rowPlan_.tiled; }
 
   void import_lanes(const LaneState& in) override {
     const int k = in.count;
@@ -1116,6 +1340,18 @@ class RunImpl final : public DeviceRun {
     int* d = buf_.alloc<int>(v.size());
     copy_h2d(d, v.data(), v.size());
     return d;
+  }
+  // Device copies of a plan's permutation and tiles.
+  void upload_plan(PlanHost& h) {
+    if (!h.perm.empty()) h.dPerm = upload_int(h.perm);
+    h.plan.perm = h.dPerm;
+    if (h.tiled) {
+      h.tplan.perm = upload_int(h.tperm);
+      h.tplan.tileRow = upload_int(h.tileRow);
+      h.tplan.tileCol = upload_int(h.tileCol);
+      h.tplan.ucols = upload_int(h.ucols.empty() ? std::vector<int>{0} : h.ucols);
+      h.dLoc = upload_int(h.loc);
+    }
   }
   double* upload_dbl(const std::vector<double>& v) {
     double* d = buf_.alloc<double>(v.size());
@@ -1174,12 +1410,24 @@ class RunImpl final : public DeviceRun {
     launches++;
   }
   void launch_col(bool check, int tloc) {
+    if (!check && colPlan_.tiled) {
+      hipLaunchKernelGGL(k_.colTiled, dim3(colPlan_.tblocks), dim3(kTileWG), 0, stream_, colPlan_.tplan, tPtr_,
+                         colPlan_.dLoc, tVal_, y_, yp_, x_, xp_, x0_, c_, lc_, lo_, up_, lb_, st_, tloc, s_.rho,
+                         colPart_, 0);
+      return;
+    }
     const int cr = colPlan_.plan.rpw - 1;
     hipLaunchKernelGGL(check ? k_.colChk[cr] : k_.col[cr], dim3(colPlan_.blocks), dim3(kWG), 0, stream_, colPlan_.plan, tPtr_,
                        tIdx_, tVal_, y_, yp_, x_, xp_, x0_, c_, lc_, lo_, up_, lb_, st_, tloc, s_.rho, colPart_,
                        colPlan_.blocks);
   }
   void launch_row(bool check, int tloc) {
+    if (!check && rowPlan_.tiled) {
+      hipLaunchKernelGGL(k_.rowTiled, dim3(rowPlan_.tblocks), dim3(kTileWG), 0, stream_, rowPlan_.tplan, aPtr_,
+                         rowPlan_.dLoc, aVal_, aVal64_, xp_, y_, yp_, y0_, ax_, axp_, ax0_, rl_, ru_, rl64_, ru64_,
+                         lr_, rfac_, st_, tloc, s_.rho, rowPart_, 0);
+      return;
+    }
     const int rr = rowPlan_.plan.rpw - 1;
     hipLaunchKernelGGL(check ? k_.rowChk[rr] : k_.row[rr], dim3(rowPlan_.blocks), dim3(kWG), 0, stream_, rowPlan_.plan, aPtr_,
                        aIdx_, aVal_, aVal64_, xp_, y_, yp_, y0_, ax_, axp_, ax0_, rl_, ru_, rl64_, ru64_, lr_, rfac_,

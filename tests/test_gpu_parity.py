@@ -354,6 +354,34 @@ def test_streamed_family_equals_fixed_batches(gpu):
           f"max case {max(r.iterations for r in streamed)}")
 
 
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32])
+def test_tiled_rows_match_one_row_per_wavefront(gpu, prec):
+    """Width 64 with the rows in LDS-staged tiles (row_map 3): each row's
+    entries are summed in the order of the one-row-per-wavefront mapping
+    (row_map 1), so every case ends with identical iterates.  On the
+    log-normal LP the longest rows of A (155 entries) exceed the FP64 staging
+    budget (96 columns), and that side keeps the plain plan."""
+    rng = np.random.default_rng(12)
+    small = P.gen_random_lp(11, 120, 240, 4.0, 0.0)
+    wide = P.gen_random_lp(2, 3000, 6000, 5.12, 1.0)
+    for lp, tiled_rows in ((small, True), (wide, prec == P.FP32)):
+        costs = lp.c[None, :] + rng.uniform(0.0, 0.2, (64, lp.n))
+        runs = []
+        for row_map in (1, 3):
+            o = P.default_options(tolerance=1e-7, precision=prec, row_map=row_map, xcd_remap=1,
+                                  max_iterations=20000, compact=0)
+            runs.append(P.solve_batch(lp, costs, o))
+        (plain, st1, _), (tiled, st3, _) = runs
+        assert (st1["col_row_map"], st1["row_row_map"]) == (1, 1)
+        assert (st3["col_row_map"], st3["row_row_map"]) == (3, 3 if tiled_rows else 1)
+        for q in range(64):
+            assert tiled[q].status == plain[q].status and tiled[q].iterations == plain[q].iterations, q
+            assert tiled[q].objective == plain[q].objective, q
+            assert np.array_equal(tiled[q].x, plain[q].x) and np.array_equal(tiled[q].y, plain[q].y), q
+        print(f"m={lp.m} n={lp.n}: {sum(r.status == P.OPTIMAL for r in tiled)}/64 optimal, "
+              f"iterations {st3['iterations']}")
+
+
 def test_streamed_family_iteration_limit(gpu):
     """Continuous batching with an iteration limit that is not a multiple of
     the check interval.  Each case is checked on its own iteration count,
