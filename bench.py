@@ -14,8 +14,10 @@ batched solve, download, certification.  The value is certified solves per
 second over the timed steps.
 
 Before the timer: the warm-start model is fitted once on a separate sample of
-the family (PDHG solutions of other theta at a looser tolerance, POD basis,
-MLP trained on the GPU); its cost is reported as warm_start_fit_seconds.
+the family (PDHG solutions of 512 other theta at the same tolerance, the
+first 64 from a cold start and the rest warm-started from a model of those;
+POD basis, MLP trained on the GPU); its cost is reported as
+warm_start_fit_seconds.
 
 Also reported: the same batched solve from a cold start, plain
 single-instance GPU PDHG on a sample of the same instances (solves/s), the
@@ -49,22 +51,33 @@ def log(*a):
 
 
 def fit_warm_start(lp, n_train, tol, opts, steps):
-    """PDHG solutions of n_train other family parameters (batches of 64 at
-    tolerance tol), POD basis and GPU-trained MLP."""
+    """PDHG solutions of n_train other family parameters at tolerance tol,
+    POD basis and GPU-trained MLP.  The first 64 parameters are solved from a
+    cold start; a model fitted to them warm-starts the rest, which stream
+    through one batched run; the final model is fitted to all of them."""
     th, costs = P.gen_family(lp, SEED_TRAIN, n_train, 0.3)
     o = P.default_options(tolerance=tol, precision=opts.precision, max_iterations=opts.max_iterations,
-                          progress_seconds=15.0)
-    xs, ys, ws_ = [], [], []
-    for s in range(0, n_train, 64):
-        res, st, _ = P.solve_batch(lp, costs[s:s + 64], o)
-        xs += [r.x for r in res]
-        ys += [r.y for r in res]
-        ws_ += [r.primal_weight for r in res]
-        log(f"warm-start snapshots {s + len(res)}/{n_train}: {st['iterations']} iterations")
-    ws = W.WarmStart(lp, rank=32, hidden=128).fit(th, np.stack(xs), np.stack(ys), steps=steps, lr=1e-3,
-                                                 omegas=np.array(ws_))
+                          progress_seconds=15.0, row_map=opts.row_map)
+    first = min(64, n_train)
+    res, st, _ = P.solve_batch(lp, costs[:first], o)
+    log(f"warm-start snapshots {first}/{n_train} (cold start): {st['iterations']} iterations")
+
+    def fit(k, rs):
+        return W.WarmStart(lp, rank=32, hidden=128).fit(th[:k], np.stack([r.x for r in rs]),
+                                                       np.stack([r.y for r in rs]), steps=steps, lr=1e-3,
+                                                       omegas=np.array([r.primal_weight for r in rs]))
+
+    if n_train > first:
+        ws0 = fit(first, res)
+        x0, y0 = ws0.predict(th[first:])
+        rest, st, _ = P.solve_batch(lp, costs[first:], o, x0=x0, y0=y0, primal_weight=ws0.primal_weights(th[first:]))
+        log(f"warm-start snapshots {n_train}/{n_train} (warm start from the first {first}): "
+            f"{st['iterations']} iterations")
+        res = list(res) + list(rest)
+    ws = fit(n_train, res)
     log(f"warm-start MLP: loss {ws.losses[0]:.3g} -> {ws.losses[-1]:.3g}, POD energy kept "
-        f"{ws.energy[:ws.used].sum() / max(ws.energy.sum(), 1e-300):.6f}")
+        f"{ws.energy[:ws.used].sum() / max(ws.energy.sum(), 1e-300):.6f}, "
+        f"{sum(r.status == P.OPTIMAL for r in res)}/{n_train} training solves converged")
     return ws
 
 
@@ -165,9 +178,11 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--precision", choices=["fp32", "fp64", "fp32auto"], default="fp64")
+    ap.add_argument("--row-map", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="row mapping of the iteration kernels: 0 autotuned, 1 buckets, 2 one mapping, 3 LDS tiles")
     ap.add_argument("--cases", type=int, default=128, help="held-out family cases solved per step")
     ap.add_argument("--cold", action="store_true", help="no learned warm start (batched cold start only)")
-    ap.add_argument("--train", type=int, default=64, help="family parameters solved to fit the warm start")
+    ap.add_argument("--train", type=int, default=512, help="family parameters solved to fit the warm start")
     ap.add_argument("--train-tol", type=float, default=1e-6, help="tolerance of the training solves")
     ap.add_argument("--train-steps", type=int, default=3000, help="Adam steps of the MLP")
     ap.add_argument("--no-cold-batch", action="store_true", help="skip the cold-start batched comparison run")
@@ -214,7 +229,7 @@ def main():
     K = args.cases
     thetas, costs = P.gen_family(lp, SEED_THETA, K, 0.3)
     opts = P.default_options(tolerance=args.tol, precision=prec, max_iterations=args.max_iterations,
-                             progress_seconds=15.0)
+                             progress_seconds=15.0, row_map=args.row_map)
     log(f"workload: refinery m={lp.m} n={lp.n} nnz={lp.nnz}, {K} instances, precision {args.precision}")
 
     ws, fit_s = None, 0.0
@@ -258,7 +273,7 @@ def main():
 
     # Kernel roofline: HIP events on the run's own stream, same workload shape.
     _, kst, _ = P.solve_batch(lp, costs[:64], P.default_options(tolerance=args.tol, precision=prec, max_iterations=64 * 8,
-                                                          kernel_timing_reps=200))
+                                                          kernel_timing_reps=200, row_map=args.row_map))
     kern = {"k_col": (kst["col_kernel_bytes"], kst["col_kernel_ms"]),
             "k_row": (kst["row_kernel_bytes"], kst["row_kernel_ms"])}
     dom = max(kern, key=lambda k: kern[k][1])
@@ -305,6 +320,8 @@ def main():
         "config": {"workload": f"refinery-family: m={lp.m} n={lp.n} nnz={lp.nnz}, {K} cost cases streamed "
                                f"through a run of width 64, tol {args.tol}",
                    "cases": K, "batch_width": 64, "start": "learned warm start" if ws is not None else "cold",
+                   "warm_start_training": (f"{args.train} family parameters at rel-KKT <= {args.train_tol:g}"
+                                           if ws is not None else None),
                    "iterations": gpu_iters, "median_instance_iterations": float(np.median(inst_iters)),
                    "certified": certified, "solved": args.steps * K},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -52,12 +52,12 @@ struct RowPlan {
 };
 
 // Tiles: a 1024-thread workgroup (16 wavefronts, one row each) per tile and
-// the LDS it stages the tile's columns in; two such workgroups fill a CU's 32
-// wavefront slots.
+// the LDS it stages the tile's columns in (128 FP64 columns); two such
+// workgroups fill a CU's 32 wavefront slots and 128 of its 160 KB of LDS.
 constexpr int kTileWaves = 16;
 constexpr int kTileWG = 64 * kTileWaves;
 constexpr int kTileRows = kTileWaves;
-constexpr int kTileBytes = 48 * 1024;
+constexpr int kTileBytes = 64 * 1024;
 template <typename T>
 constexpr int kTileCols = kTileBytes / (64 * static_cast<int>(sizeof(T)));
 

@@ -360,7 +360,7 @@ def test_tiled_rows_match_one_row_per_wavefront(gpu, prec):
     entries are summed in the order of the one-row-per-wavefront mapping
     (row_map 1), so every case ends with identical iterates.  On the
     log-normal LP the longest rows of A (155 entries) exceed the FP64 staging
-    budget (96 columns), and that side keeps the plain plan."""
+    budget (128 columns), and that side keeps the plain plan."""
     rng = np.random.default_rng(12)
     small = P.gen_random_lp(11, 120, 240, 4.0, 0.0)
     wide = P.gen_random_lp(2, 3000, 6000, 5.12, 1.0)
