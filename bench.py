@@ -63,9 +63,12 @@ def fit_warm_start(lp, n_train, tol, opts, steps):
     log(f"warm-start snapshots {first}/{n_train} (cold start): {st['iterations']} iterations")
 
     def fit(k, rs):
-        return W.WarmStart(lp, rank=32, hidden=128).fit(th[:k], np.stack([r.x for r in rs]),
-                                                       np.stack([r.y for r in rs]), steps=steps, lr=1e-3,
-                                                       omegas=np.array([r.primal_weight for r in rs]))
+        # Only converged solutions are snapshots: a case stopped at the
+        # iteration limit is not a solution of its parameter.
+        ok = [q for q in range(k) if rs[q].status == P.OPTIMAL]
+        return W.WarmStart(lp, rank=32, hidden=128).fit(th[ok], np.stack([rs[q].x for q in ok]),
+                                                       np.stack([rs[q].y for q in ok]), steps=steps, lr=1e-3,
+                                                       omegas=np.array([rs[q].primal_weight for q in ok]))
 
     if n_train > first:
         ws0 = fit(first, res)
@@ -77,7 +80,7 @@ def fit_warm_start(lp, n_train, tol, opts, steps):
     ws = fit(n_train, res)
     log(f"warm-start MLP: loss {ws.losses[0]:.3g} -> {ws.losses[-1]:.3g}, POD energy kept "
         f"{ws.energy[:ws.used].sum() / max(ws.energy.sum(), 1e-300):.6f}, "
-        f"{sum(r.status == P.OPTIMAL for r in res)}/{n_train} training solves converged")
+        f"fitted to the {sum(r.status == P.OPTIMAL for r in res)}/{n_train} converged training solves")
     return ws
 
 
@@ -280,10 +283,12 @@ def main():
     dbytes, dms = kern[dom]
     achieved = dbytes / (dms * 1e-3) / 1e9
     kprec = "fp64" if prec == P.FP64 else "fp32"
+    # Row mapping the tuner chose for that kernel: LDS-staged tiles (3) or rows.
+    tiled = kst["col_row_map" if dom == "k_col" else "row_row_map"] == 3
     traffic = None
     tf = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get(f"{dom}_{kprec}")
+        traffic = json.load(open(tf)).get(f"{dom}{'_tiled' if tiled else ''}_{kprec}")
     log(f"kernels: k_col {kern['k_col'][1] * 1e3:.1f} us ({kern['k_col'][0] / kern['k_col'][1] / 1e6:.0f} GB/s), "
         f"k_row {kern['k_row'][1] * 1e3:.1f} us ({kern['k_row'][0] / kern['k_row'][1] / 1e6:.0f} GB/s)")
 
@@ -324,7 +329,8 @@ def main():
                                            if ws is not None else None),
                    "iterations": gpu_iters, "median_instance_iterations": float(np.median(inst_iters)),
                    "certified": certified, "solved": args.steps * K},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": dom + ("_tiled" if tiled else ""), "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "algorithmic_bytes": dbytes,
                      "launch_us": dms * 1e3},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
