@@ -545,7 +545,13 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       // cheaper (the row mapping changes with the width).
       const int newPrec = toFp64 ? PDHG_FP64 : runPrec;
       int newB = B;
-      if ((o.compact && B > 1 && remaining <= B / 2 && pending.empty()) || toFp64) {
+      const bool mayNarrow = o.compact && B > 1 && remaining <= B / 2 && pending.empty();
+      if (!autoMap) {
+        // A fixed mapping keeps runs reproducible: the width follows from the
+        // live count alone (the narrowest power of two that holds them), not
+        // from timings.
+        if (mayNarrow) newB = next_pow2(remaining);
+      } else if (mayNarrow || toFp64) {
         double best = toFp64 ? kInf : width_cost(B, runPrec);
         for (int W = next_pow2(remaining); W <= B; W *= 2) {
           if ((!o.compact || !pending.empty()) && W != B) continue;

@@ -225,7 +225,14 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
   T accs[2];
   double accs64[2] = {0.0, 0.0};
   lane_rows<LOGB, RPW, T, SPL>(plan, bid, s, ptr, idx, val, xp, b, rows, accs);
-  if (CHECK) lane_rows<LOGB, RPW, double, SPL>(plan, bid, s, ptr, idx, val64, xp, b, rows, accs64);
+  if constexpr (CHECK) {
+    if constexpr (sizeof(T) == 8) {  // FP64 iterates: val is val64, the same sums
+      accs64[0] = accs[0];
+      accs64[1] = accs[1];
+    } else {
+      lane_rows<LOGB, RPW, double, SPL>(plan, bid, s, ptr, idx, val64, xp, b, rows, accs64);
+    }
+  }
   double q[4] = {0.0, 0.0, 0.0, 0.0};
   const long long it = st.iter[b] + tloc;
   for (int rr = 0; rr < 2; ++rr) {
@@ -372,20 +379,23 @@ __global__ __launch_bounds__(kWG) void k_finalize(Inst st, const double* __restr
   const int b = blockIdx.x, B = gridDim.x;
   if (st.done[b]) return;
   constexpr int NQ = 9;  // col 0..4, row 0..3
-  double v[NQ];
+  // Each thread sums partials k = tid, tid + kWG, ... of every quantity in
+  // that order; the quantities share one loop so their loads are in flight
+  // together.
+  double v[NQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const double* pc = colPart + static_cast<size_t>(b) * nblkC;
+  const size_t sc5 = static_cast<size_t>(B) * nblkC;
+#pragma unroll 2
+  for (int k = threadIdx.x; k < nblkC; k += kWG) {
 #pragma unroll
-  for (int q = 0; q < 5; ++q) {
-    const double* p = colPart + (static_cast<size_t>(q) * B + b) * nblkC;
-    double s = 0;
-    for (int k = threadIdx.x; k < nblkC; k += kWG) s += p[k];
-    v[q] = s;
+    for (int q = 0; q < 5; ++q) v[q] += pc[q * sc5 + k];
   }
+  const double* pr = rowPart + static_cast<size_t>(b) * nblkR;
+  const size_t sr4 = static_cast<size_t>(B) * nblkR;
+#pragma unroll 2
+  for (int k = threadIdx.x; k < nblkR; k += kWG) {
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const double* p = rowPart + (static_cast<size_t>(q) * B + b) * nblkR;
-    double s = 0;
-    for (int k = threadIdx.x; k < nblkR; k += kWG) s += p[k];
-    v[5 + q] = s;
+    for (int q = 0; q < 4; ++q) v[5 + q] += pr[q * sr4 + k];
   }
   __shared__ double sh[NQ][kWG / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

@@ -4,6 +4,8 @@ Bar: SpMV within FP32 rounding of the FP64 product; full solves reach the
 requested relative KKT (re-evaluated independently in FP64 on the host) and
 their objectives agree with the oracle's and with the closed-form optimum.
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -380,6 +382,34 @@ def test_tiled_rows_match_one_row_per_wavefront(gpu, prec):
             assert np.array_equal(tiled[q].x, plain[q].x) and np.array_equal(tiled[q].y, plain[q].y), q
         print(f"m={lp.m} n={lp.n}: {sum(r.status == P.OPTIMAL for r in tiled)}/64 optimal, "
               f"iterations {st3['iterations']}")
+
+
+_REPRO = """
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+import pdhg_amd as P
+lp = P.gen_random_lp(11, 120, 240, 4.0, 0.0)
+costs = lp.c[None, :] + np.random.default_rng(5).uniform(0.0, 0.2, (48, lp.n))
+o = P.default_options(tolerance=1e-7, precision=P.FP64, lanes_per_row=4, row_map=1, xcd_remap=1)
+res, st, _ = P.solve_batch(lp, costs, o)
+np.save(sys.argv[2], np.concatenate([np.stack([r.x for r in res]).ravel(), [r.iterations for r in res]]))
+"""
+
+
+def test_fixed_mapping_is_reproducible_across_processes(gpu, tmp_path):
+    """With a fixed row mapping the run's widths follow from the live counts
+    alone (compaction to the narrowest power of two), so two processes solving
+    the same family produce identical bits; autotuned mappings and
+    cost-based compaction depend on timings."""
+    import subprocess
+    import sys as _sys
+    pkg = str(Path(P.__file__).resolve().parent)
+    outs = []
+    for k in range(2):
+        f = tmp_path / f"run{k}.npy"
+        subprocess.run([_sys.executable, "-c", _REPRO, pkg, str(f)], check=True, timeout=300)
+        outs.append(np.load(f))
+    assert np.array_equal(outs[0], outs[1])
 
 
 def test_streamed_family_iteration_limit(gpu):
