@@ -794,8 +794,11 @@ const DeviceInfo& device_info(int device) {
   d.arch = p.gcnArchName;
   d.computeUnits = p.multiProcessorCount;
   d.memoryBytes = p.totalGlobalMem;
-  // memoryClockRate is in kHz; bus width in bits; DDR.
+  // memoryClockRate (kHz) x bus width x 2 is the DDR formula; it reads half
+  // of HBM3E's rate on gfx950 (4.1 TB/s), so that part reports its published
+  // peak (MI350 series, 8 stacks: 8.0 TB/s).
   d.peakBandwidthGBs = 2.0 * p.memoryClockRate * 1e3 * (p.memoryBusWidth / 8.0) / 1e9;
+  if (d.arch.rfind("gfx950", 0) == 0) d.peakBandwidthGBs = 8000.0;
   if (d.arch.rfind("gfx950", 0) != 0) {
     d.reason = "device " + d.arch + " is not gfx950 (this build carries gfx950 code objects only)";
     return d;
