@@ -47,6 +47,7 @@ constexpr int kTaskGroups = 4;
 // XCD's L2).  Each queue's counter has its own 128-byte line.
 struct Queues {
   int beg[9];
+  int direct;  // 1: one workgroup per task (task = blockIdx), no tickets
 };
 constexpr int kQueueStride = 32;  // ints between counters
 constexpr int kQueueSet = 8 * kQueueStride;
@@ -70,6 +71,14 @@ __device__ inline int next_ticket(int* counters, const Queues& qs) {
   __syncthreads();
   return ticket;
 }
+
+// The first and the following tasks of a workgroup: tickets from its queue,
+// or in a direct launch (a grid of exactly beg[8] workgroups) its own task.
+__device__ inline int first_task(int* counters, const Queues& qs) {
+  if (qs.direct) return static_cast<int>(blockIdx.x) < qs.beg[8] ? static_cast<int>(blockIdx.x) : -1;
+  return next_ticket(counters, qs);
+}
+__device__ inline int next_task(int* counters, const Queues& qs) { return qs.direct ? -1 : next_ticket(counters, qs); }
 
 struct PSlot {
   int row, sub, logS;
@@ -95,7 +104,7 @@ __global__ __launch_bounds__(kWG) void k_pcol(const PTask* __restrict__ tasks, i
                                               T* __restrict__ xp, T* __restrict__ x0, const T* __restrict__ c,
                                               const T* __restrict__ lo, const T* __restrict__ up, Inst st, int tloc,
                                               double rho, double* __restrict__ part) {
-  for (int tk; (tk = next_ticket(queue, qs)) >= 0;) {
+  for (int tk = first_task(queue, qs); tk >= 0; tk = next_task(queue, qs)) {
     const PTask task = tasks[tk];
     const int b = task.inst;
     if (st.done[b]) continue;
@@ -144,7 +153,7 @@ __global__ __launch_bounds__(kWG) void k_prow(const PTask* __restrict__ tasks, i
                                               const double* __restrict__ rl64, const double* __restrict__ ru64,
                                               const double* __restrict__ rfac, Inst st, int tloc, double rho,
                                               double* __restrict__ part) {
-  for (int tk; (tk = next_ticket(queue, qs)) >= 0;) {
+  for (int tk = first_task(queue, qs); tk >= 0; tk = next_task(queue, qs)) {
     const PTask task = tasks[tk];
     const int b = task.inst;
     if (st.done[b]) continue;
@@ -213,7 +222,7 @@ __global__ __launch_bounds__(kWG) void k_pcolcheck(const PTask* __restrict__ tas
                                                    const double* __restrict__ lo64, const double* __restrict__ up64,
                                                    const double* __restrict__ cfac, Inst st,
                                                    double* __restrict__ part) {
-  for (int tk; (tk = next_ticket(queue, qs)) >= 0;) {
+  for (int tk = first_task(queue, qs); tk >= 0; tk = next_task(queue, qs)) {
     const PTask task = tasks[tk];
     if (st.done[task.inst]) continue;
     double q[3] = {0.0, 0.0, 0.0};
@@ -294,7 +303,7 @@ __global__ __launch_bounds__(kWG) void k_pspmv(const PTask* __restrict__ tasks, 
                                                int* queue, Queues qs, const int* __restrict__ ptr,
                                                const int* __restrict__ idx, const T* __restrict__ val,
                                                const T* __restrict__ in, T* __restrict__ out) {
-  for (int tk; (tk = next_ticket(queue, qs)) >= 0;) {
+  for (int tk = first_task(queue, qs); tk >= 0; tk = next_task(queue, qs)) {
     const PTask task = tasks[tk];
     const int ng = task_groups(task);
     for (int g = 0; g < ng; ++g) {
@@ -344,19 +353,41 @@ void build_tasks(const std::vector<int>& ptr, const std::vector<int>& rowOff, in
 // The live part of a task list: tasks of instances not done, in order, with
 // their per-instance ranges (empty for finished instances) and the split of
 // the list over the eight queues.
+// When the live tasks cut into single workgroup groups number at most
+// maxSplit, each is cut so: a few live instances then spread over many
+// workgroups instead of a few workgroups walking their groups in turn.
 void live_tasks(const std::vector<PTask>& all, const std::vector<int>& allBeg, const std::vector<int>& done,
-                std::vector<PTask>& out, std::vector<int>& beg, Queues& qs) {
+                std::vector<PTask>& out, std::vector<int>& beg, Queues& qs, int maxSplit) {
   const int K = static_cast<int>(allBeg.size()) - 1;
+  auto groups = [](const PTask& t) { return (t.count + (kWG >> t.logS) - 1) / (kWG >> t.logS); };
+  long long pieces = 0;
+  for (int b = 0; b < K; ++b)
+    if (done.empty() || !done[b])
+      for (int i = allBeg[b]; i < allBeg[b + 1]; ++i) pieces += groups(all[i]);
+  const bool split = pieces <= maxSplit;
   out.clear();
   beg.assign(K + 1, 0);
   for (int b = 0; b < K; ++b) {
     beg[b] = static_cast<int>(out.size());
     if (!done.empty() && done[b]) continue;
-    out.insert(out.end(), all.begin() + allBeg[b], all.begin() + allBeg[b + 1]);
+    for (int i = allBeg[b]; i < allBeg[b + 1]; ++i) {
+      if (!split) {
+        out.push_back(all[i]);
+        continue;
+      }
+      const int per = kWG >> all[i].logS;
+      for (int g = 0; g * per < all[i].count; ++g) {
+        PTask t = all[i];
+        t.start += g * per;
+        t.count = std::min(per, all[i].count - g * per);
+        out.push_back(t);
+      }
+    }
   }
   beg[K] = static_cast<int>(out.size());
   const int n = beg[K];
   for (int q = 0; q <= 8; ++q) qs.beg[q] = static_cast<int>((static_cast<long long>(n) * q) / 8);
+  qs.direct = split ? 1 : 0;
 }
 
 template <typename T>
@@ -490,7 +521,7 @@ class PackedRun {
   }
 
   void start() {
-    hipLaunchKernelGGL(k_pspmv<T>, dim3(grid_), dim3(kWG), 0, s_, rowTasks_, nRow_, rowPerm_, queue_ + 3 * kQueueSet,
+    hipLaunchKernelGGL(k_pspmv<T>, dim3(rowGrid_), dim3(kWG), 0, s_, rowTasks_, nRow_, rowPerm_, queue_ + 3 * kQueueSet,
                        rowQ_, aPtr_, aIdx_, aVal_, x_, ax_);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(ax0_, ax_, sizeof(T) * M_, hipMemcpyDeviceToDevice, s_));
@@ -550,9 +581,9 @@ class PackedRun {
   // current lists (the kernels skip them anyway, but every skipped task still
   // costs a ticket); the graph is captured again for the new lists.
   void drop_finished(const std::vector<int>& done) {
-    int live = 0;
+    int live = 0;  // tasks of the current lists that belong to live instances
     for (int b = 0; b < K_; ++b)
-      if (!done[b]) live += (colBegAll_[b + 1] - colBegAll_[b]) + (rowBegAll_[b + 1] - rowBegAll_[b]);
+      if (!done[b]) live += (colBeg_[b + 1] - colBeg_[b]) + (rowBeg_[b + 1] - rowBeg_[b]);
     if (live == 0 || 2 * live > nCol_ + nRow_) return;
     set_tasks(done);
     if (exec_) {
@@ -572,17 +603,23 @@ class PackedRun {
   void set_tasks(const std::vector<int>& done) {
     std::vector<PTask> ct, rt;
     std::vector<int> cb, rb;
-    live_tasks(colAll_, colBegAll_, done, ct, cb, colQ_);
-    live_tasks(rowAll_, rowBegAll_, done, rt, rb, rowQ_);
+    // The task buffers and partials hold as many tasks as the full lists.
+    live_tasks(colAll_, colBegAll_, done, ct, cb, colQ_, std::min<int>(maxGrid_, colAll_.size()));
+    live_tasks(rowAll_, rowBegAll_, done, rt, rb, rowQ_, std::min<int>(maxGrid_, rowAll_.size()));
     nCol_ = static_cast<int>(ct.size());
     nRow_ = static_cast<int>(rt.size());
     // No more workgroups than tasks: a launch over a few live instances
     // then starts a few workgroups instead of the full persistent grid.
     grid_ = std::min(maxGrid_, std::max(8, (std::max(nCol_, nRow_) + 7) / 8 * 8));
+    // Direct launches (few live tasks): exactly one workgroup per task.
+    colGrid_ = colQ_.direct ? std::max(1, nCol_) : grid_;
+    rowGrid_ = rowQ_.direct ? std::max(1, nRow_) : grid_;
     if (!ct.empty()) putv(colTasks_, ct);
     if (!rt.empty()) putv(rowTasks_, rt);
     putv(dColBeg_, cb);
     putv(dRowBeg_, rb);
+    colBeg_ = cb;
+    rowBeg_ = rb;
   }
 
   void enqueue(int iters) {
@@ -590,13 +627,13 @@ class PackedRun {
       const bool chk = t == iters - 1;
       auto colK = chk ? k_pcol<T, true> : k_pcol<T, false>;
       auto rowK = chk ? k_prow<T, true> : k_prow<T, false>;
-      hipLaunchKernelGGL(colK, dim3(grid_), dim3(kWG), 0, s_, colTasks_, nCol_, colPerm_, queue_, colQ_, tPtr_,
+      hipLaunchKernelGGL(colK, dim3(colGrid_), dim3(kWG), 0, s_, colTasks_, nCol_, colPerm_, queue_, colQ_, tPtr_,
                          tIdx_, tVal_, y_, yp_, x_, xp_, x0_, c_, lo_, up_, st_, t, rho_, colPart_);
-      hipLaunchKernelGGL(rowK, dim3(grid_), dim3(kWG), 0, s_, rowTasks_, nRow_, rowPerm_, queue_ + kQueueSet, rowQ_,
+      hipLaunchKernelGGL(rowK, dim3(rowGrid_), dim3(kWG), 0, s_, rowTasks_, nRow_, rowPerm_, queue_ + kQueueSet, rowQ_,
                          aPtr_, aIdx_, aVal_, aVal64_, xp_, y_, yp_, y0_, ax_, axp_, ax0_, rl_, ru_, rl64_, ru64_,
                          rfac_, st_, t, rho_, rowPart_);
     }
-    hipLaunchKernelGGL(k_pcolcheck<T>, dim3(grid_), dim3(kWG), 0, s_, colTasks_, nCol_, colPerm_,
+    hipLaunchKernelGGL(k_pcolcheck<T>, dim3(colGrid_), dim3(kWG), 0, s_, colTasks_, nCol_, colPerm_,
                        queue_ + 2 * kQueueSet, colQ_, tPtr_,
                        tIdx_, tVal64_, yp_, xp_, c64_, lo64_, up64_, cfac_, st_, colPart_);
     hipLaunchKernelGGL(k_pfinalize, dim3(K_), dim3(kWG), 0, s_, st_, colPart_, nCol_, dColBeg_, rowPart_, nRow_,
@@ -628,7 +665,8 @@ class PackedRun {
   int M_, N_, K_, checkEvery_;
   double rho_;
   bool useGraph_;
-  int grid_ = 8, maxGrid_ = 8, nCol_ = 0, nRow_ = 0;
+  int grid_ = 8, maxGrid_ = 8, nCol_ = 0, nRow_ = 0, colGrid_ = 8, rowGrid_ = 8;
+  std::vector<int> colBeg_, rowBeg_;  // per-instance ranges of the current task lists
   double tol_ = 1e-6;
   long long maxIter_ = 1LL << 62;
   hipStream_t s_ = nullptr;
