@@ -19,7 +19,8 @@ first 64 from a cold start and the rest warm-started from a model of those;
 POD basis, MLP trained on the GPU); its cost is reported as
 warm_start_fit_seconds.
 
-Also reported: the same batched solve from a cold start, plain
+Also reported: config C4's 256 uneven LPs solved in one packed run of
+persistent kernels (packed_uneven), the same batched solve from a cold start, plain
 single-instance GPU PDHG on a sample of the same instances (solves/s), the
 per-launch roofline of the dominant iteration kernel (algorithmic bytes /
 HIP-event time, HBM peak 8 TB/s), and the reference's own 16-thread CPU PDHG
@@ -43,7 +44,7 @@ import pdhg_amd as P  # noqa: E402
 import warmstart as W  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
-SEED_LP, SEED_THETA, SEED_TRAIN = 3, 4, 5
+SEED_LP, SEED_THETA, SEED_TRAIN, SEED_PACKED = 3, 4, 5, 7
 
 
 def log(*a):
@@ -189,6 +190,7 @@ def main():
     ap.add_argument("--train-tol", type=float, default=1e-6, help="tolerance of the training solves")
     ap.add_argument("--train-steps", type=int, default=3000, help="Adam steps of the MLP")
     ap.add_argument("--no-cold-batch", action="store_true", help="skip the cold-start batched comparison run")
+    ap.add_argument("--no-packed", action="store_true", help="skip the packed run of 256 uneven LPs")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--plain-sample", type=int, default=2, help="instances solved one by one for the plain baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -274,6 +276,23 @@ def main():
                 "median_instance_iterations": float(np.median([r.iterations for r in res_c]))}
         log(f"batched cold start: {el:.2f} s, {st_c['iterations']} iterations, {ok_c.sum()}/{K} certified")
 
+    # Config C4's uneven batch: 256 unrelated LPs (the config-0 generator, nnz
+    # log-uniform in [2e3, 2e5]) in one packed run of persistent kernels; the
+    # time includes certifying every answer.
+    packed = None
+    if not args.no_packed:
+        lps = P.gen_uneven_batch(SEED_PACKED, 256)
+        t = time.perf_counter()
+        res_p, st_p = P.solve_packed(lps, P.default_options(tolerance=args.tol, precision=prec,
+                                                            max_iterations=args.max_iterations,
+                                                            progress_seconds=15.0))
+        cert_p = sum(1 for q, r in zip(lps, res_p)
+                     if r.status == P.OPTIMAL and P.certify(q, r.x, r.y, r.objective)["accepted"])
+        el = time.perf_counter() - t
+        packed = {"lps": len(lps), "nnz": int(sum(q.nnz for q in lps)), "seconds": el, "certified": cert_p,
+                  "solves_per_s": cert_p / el, "max_instance_iterations": int(max(r.iterations for r in res_p))}
+        log(f"packed uneven batch: {cert_p}/{len(lps)} certified in {el:.2f} s")
+
     # Kernel roofline: HIP events on the run's own stream, same workload shape.
     _, kst, _ = P.solve_batch(lp, costs[:64], P.default_options(tolerance=args.tol, precision=prec, max_iterations=64 * 8,
                                                           kernel_timing_reps=200, row_map=args.row_map))
@@ -337,6 +356,7 @@ def main():
         "plain_gpu_solves_per_s": plain,
         "vs_plain_gpu": (value / plain) if plain else None,
         "batched_cold": cold,
+        "packed_uneven": packed,
         "warm_start_fit_seconds": fit_s if ws is not None else None,
         "max_rel_kkt": max(kkt),
         "max_certified_gap": max(gaps),

@@ -558,6 +558,44 @@ class PackedRun {
     HIP_OK(hipStreamSynchronize(s_));
   }
 
+  // Iterates, anchors, activities and restart state of slot q (columns
+  // col0 .. col0+n, rows row0 .. row0+m) as a one-lane LaneState.
+  void export_slot(int q, int col0, int n, int row0, int m, LaneState& ls) {
+    auto get = [&](const T* d, int off, int len, std::vector<double>& out) {
+      std::vector<T> t(len);
+      HIP_OK(hipMemcpyAsync(t.data(), d + off, len * sizeof(T), hipMemcpyDeviceToHost, s_));
+      HIP_OK(hipStreamSynchronize(s_));
+      out.assign(t.begin(), t.end());
+    };
+    get(x_, col0, n, ls.x);
+    get(x0_, col0, n, ls.x0);
+    get(xp_, col0, n, ls.xp);
+    get(y_, row0, m, ls.y);
+    get(y0_, row0, m, ls.y0);
+    get(yp_, row0, m, ls.yp);
+    get(ax_, row0, m, ls.ax);
+    get(ax0_, row0, m, ls.ax0);
+    get(axp_, row0, m, ls.axp);
+    auto one = [&](const auto* d, auto& out) {
+      using U = std::remove_cv_t<std::remove_pointer_t<decltype(d)>>;
+      U v{};
+      HIP_OK(hipMemcpyAsync(&v, d + q, sizeof(U), hipMemcpyDeviceToHost, s_));
+      HIP_OK(hipStreamSynchronize(s_));
+      out.assign(1, v);
+    };
+    one(st_.tau, ls.tau);
+    one(st_.sigma, ls.sigma);
+    one(st_.omega, ls.omega);
+    one(st_.fpR, ls.fpR);
+    one(st_.fpPrev, ls.fpPrev);
+    one(st_.iter, ls.iter);
+    one(st_.since, ls.since);
+    one(st_.restartAt, ls.restartAt);
+    one(st_.applyAt, ls.applyAt);
+    one(st_.restarts, ls.restarts);
+    ls.count = 1;
+  }
+
   void download(bool primal, std::vector<double>& out) {
     const int len = primal ? N_ : M_;
     std::vector<T> t(len);
@@ -709,6 +747,9 @@ struct PackedOutcome {
   long long it = 0, launches = 0, graphLaunches = 0;
   bool expired = false, stalled = false;
   double kernel = 0;
+  // Live slots handed to single-LP runs, with their state.
+  std::vector<int> handed;
+  std::vector<LaneState> handedState;
 };
 
 // xw / yw / ww (indexed by instance): a starting point in the scaled space and
@@ -807,7 +848,7 @@ PackedData pack(const std::vector<Model>& lps, const std::vector<Prepared>& P, c
 // instances are, the run ends with stalled = true.
 template <typename T>
 PackedOutcome run_packed(const PackedData& d, const pdhg_options& o, long long maxIter, Clock::time_point t0,
-                         bool stallRule) {
+                         bool stallRule, int handoff) {
   const int K = static_cast<int>(d.ids.size());
   PackedRun<T> run(o.device, d.A, d.AT, d.rowOff, d.colOff, d.rfac, d.cfac, o.lanes_per_row > 0 ? o.lanes_per_row : 4,
                    o.check_every, o.reflection, o.use_graph != 0);
@@ -851,6 +892,19 @@ PackedOutcome run_packed(const PackedData& d, const pdhg_options& o, long long m
       r.stalled = true;
       break;
     }
+    if (remaining <= handoff && remaining < K) {
+      // A few LPs left: each continues in a single-LP run, which iterates
+      // faster than the packed kernels over a handful of tasks.
+      for (int q = 0; q < K; ++q) {
+        if (r.done[q]) continue;
+        LaneState ls;
+        run.export_slot(q, d.colOff[q], d.colOff[q + 1] - d.colOff[q], d.rowOff[q], d.rowOff[q + 1] - d.rowOff[q],
+                        ls);
+        r.handed.push_back(q);
+        r.handedState.push_back(std::move(ls));
+      }
+      break;
+    }
     run.drop_finished(r.done);
   }
   r.kernel = run.stop();
@@ -887,6 +941,119 @@ void write_results(const PackedData& d, const PackedOutcome& r, const std::vecto
   }
 }
 
+// The handed-off slots of a packed run, each continued in its own single-LP
+// run (FP64 iterates, one lane) from the exported state, with the same
+// options and its own iteration limit; the runs' blocks are interleaved on
+// their own streams.  Overwrites those slots' results; returns kernel seconds
+// and adds the iterations the runs took beyond the packed run's.
+double finish_handed(const PackedData& d, const PackedOutcome& r, const std::vector<Prepared>& P,
+                     const pdhg_options& o, Clock::time_point t0, pdhg_result* results, long long* extraIters) {
+  struct Slot {
+    int q, b;
+    std::unique_ptr<DeviceRun> run;
+    long long own = 0;
+    bool finished = false, limit = false;
+    CheckRecord rec{};
+    int done = 0, restarts = 0;
+    long long doneIter = -1;
+  };
+  std::vector<Slot> slots;
+  for (size_t h = 0; h < r.handed.size(); ++h) {
+    const int q = r.handed[h], b = d.ids[q];
+    const Prepared& p = P[b];
+    RunSetup rs;
+    rs.m = p.m;
+    rs.n = p.n;
+    rs.B = 1;
+    rs.precision = PDHG_FP64;
+    rs.colMap.nnzPerLane = rs.rowMap.nnzPerLane = o.lanes_per_row > 0 ? o.lanes_per_row : 4;
+    rs.rho = o.reflection;
+    rs.checkEvery = o.check_every;
+    rs.useGraph = o.use_graph != 0;
+    Slot sl;
+    sl.q = q;
+    sl.b = b;
+    sl.run = DeviceRun::create(o.device, p, rs);
+    const int c0 = d.colOff[q], r0 = d.rowOff[q];
+    auto cut = [](const std::vector<double>& v, int off, int len) {
+      return std::vector<double>(v.begin() + off, v.begin() + off + len);
+    };
+    const auto c = cut(d.c, c0, p.n), lo = cut(d.lo, c0, p.n), up = cut(d.upv, c0, p.n);
+    const auto rl = cut(d.rl, r0, p.m), ru = cut(d.ru, r0, p.m);
+    sl.run->upload(c.data(), lo.data(), up.data(), rl.data(), ru.data(), r.handedState[h].x.data(),
+                   r.handedState[h].y.data());
+    RunScalars sc;
+    sc.eta = d.eta[q];
+    sc.beta = d.beta[q];
+    sc.gamma = d.gamma[q];
+    sc.offset = d.offset[q];
+    sc.tol = o.tolerance;
+    sc.maxIter = o.max_iterations;
+    const int live = 0;
+    sl.run->init_state(sc, &d.omega[q], &d.cNorm[q], &d.rhsNorm[q], &live);
+    sl.run->import_lanes(r.handedState[h]);
+    sl.run->start(false);
+    sl.own = r.handedState[h].iter[0];
+    slots.push_back(std::move(sl));
+  }
+  double kernel = 0;
+  long long extra = 0;
+  bool expired = false;
+  for (;;) {
+    bool any = false;
+    for (Slot& sl : slots) {
+      if (sl.finished) continue;
+      const long long left = o.max_iterations - sl.own;
+      const int nIt = static_cast<int>(std::min<long long>(o.check_every - sl.own % o.check_every, left));
+      sl.run->run_block(nIt);
+      sl.own += nIt;
+      any = true;
+    }
+    if (!any) break;
+    for (Slot& sl : slots) {
+      if (sl.finished) continue;
+      sl.run->fetch(&sl.rec, &sl.done, &sl.doneIter, &sl.restarts);
+      if (sl.done || sl.own >= o.max_iterations) {
+        sl.finished = true;
+        sl.limit = !sl.done;
+      }
+    }
+    if (o.time_limit > 0 && std::chrono::duration<double>(Clock::now() - t0).count() >= o.time_limit) {
+      expired = true;
+      break;
+    }
+  }
+  for (Slot& sl : slots) {
+    kernel += sl.run->stop();
+    extra = std::max(extra, sl.own - r.it);
+    pdhg_result& out = results[sl.b];
+    const Prepared& p = P[sl.b];
+    out.status = sl.done ? PDHG_OPTIMAL : (expired && !sl.limit ? PDHG_TIME_LIMIT : PDHG_ITERATION_LIMIT);
+    out.iterations = sl.done ? sl.doneIter : sl.own;
+    out.restarts = sl.restarts;
+    const CheckRecord& c = sl.rec;
+    out.objective = c.pobj;
+    out.dual_objective = c.dobj;
+    out.rel_primal = c.relP;
+    out.rel_dual = c.relD;
+    out.rel_gap = c.gap;
+    out.relative_kkt = std::max(c.relP, std::max(c.relD, c.gap));
+    out.primal_weight = c.omega;
+    std::vector<double> xs(p.n), ys(p.m);
+    sl.run->download(0, xs.data());
+    sl.run->download(1, ys.data());
+    if (out.x)
+      for (int j = 0; j < p.n; ++j) out.x[j] = xs[j] * p.C[j] * p.beta;
+    if (out.row_dual)
+      for (int i = 0; i < p.m; ++i) out.row_dual[i] = ys[i] * p.R[i] * p.gamma;
+  }
+  *extraIters = extra;
+  return kernel;
+}
+
+// LPs still live when at most this many remain move to single-LP runs.
+constexpr int kHandoff = 4;
+
 void solve_packed_all(const std::vector<Model>& lps, const pdhg_options& o, pdhg_result* results,
                       pdhg_stats* stats) {
   const auto t0 = Clock::now();
@@ -915,11 +1082,16 @@ void solve_packed_all(const std::vector<Model>& lps, const pdhg_options& o, pdhg
   const auto tSolve = Clock::now();
   std::vector<int> restartBase(K, 0);
   PackedOutcome r;
-  if (o.precision == PDHG_FP64) r = run_packed<double>(d, o, o.max_iterations, t0, false);
-  else r = run_packed<float>(d, o, o.max_iterations, t0, o.precision == PDHG_FP32_AUTO);
+  if (o.precision == PDHG_FP64) r = run_packed<double>(d, o, o.max_iterations, t0, false, kHandoff);
+  else r = run_packed<float>(d, o, o.max_iterations, t0, o.precision == PDHG_FP32_AUTO, 0);
   write_results(d, r, P, 0, restartBase, results);
   long long iterations = r.it, launches = r.launches, graphLaunches = r.graphLaunches;
   double kernel = r.kernel;
+  if (!r.handed.empty()) {
+    long long extra = 0;
+    kernel += finish_handed(d, r, P, o, t0, results, &extra);
+    iterations += extra;
+  }
   long long switchAt = -1;
   int restartsTotal = 0;
   for (int q = 0; q < K; ++q) restartsTotal += r.restarts[q];
@@ -943,7 +1115,7 @@ void solve_packed_all(const std::vector<Model>& lps, const pdhg_options& o, pdhg
       restartBase[b] = r.restarts[q];
     }
     const PackedData d2 = pack(lps, P, live, &xw, &yw, &ww);
-    const PackedOutcome r2 = run_packed<double>(d2, o, o.max_iterations - r.it, t0, false);
+    const PackedOutcome r2 = run_packed<double>(d2, o, o.max_iterations - r.it, t0, false, 0);
     write_results(d2, r2, P, r.it, restartBase, results);
     iterations += r2.it;
     launches += r2.launches;
