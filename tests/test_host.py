@@ -85,6 +85,28 @@ def test_generator_random_lp_closed_form():
     assert np.all(lp.l == 0) and np.all(np.isinf(lp.u))
 
 
+def test_certifier_on_unbounded_columns():
+    """Config-0 LPs have x >= 0 without upper bounds, and their mixed-sign
+    equality rows imply none.  The safe bound at the generator's exact dual
+    is finite and certifies the optimum.  Moving one reduced cost just below
+    zero (as in any inexact PDHG dual) makes the rigorous bound -inf
+    (src/cert/certifier.cpp:28-118), so such answers are never accepted."""
+    lp = P.gen_random_lp(1, 200, 400, 4.0, 0.0)
+    A = lp.scipy_csc()
+    obj = float(lp.c @ lp.xstar)
+    lb, _, _ = P.safe_dual_bound(lp, lp.ystar)
+    assert np.isfinite(lb) and abs(lb - obj) <= 1e-9 * (1 + abs(obj))
+    assert P.certify(lp, lp.xstar, lp.ystar, obj)["accepted"]
+    s = lp.c - A.T @ lp.ystar
+    j = int(np.argmin(np.where(A.getnnz(axis=0) > 0, s, np.inf)))
+    i = int(A[:, j].nonzero()[0][0])
+    y = lp.ystar.copy()
+    y[i] += (s[j] + 1e-9) / A[i, j]  # (c - A^T y)_j = -1e-9
+    assert (lp.c - A.T @ y)[j] < 0
+    assert P.safe_dual_bound(lp, y)[0] == -np.inf
+    assert not P.certify(lp, lp.xstar, y, obj)["accepted"]
+
+
 def test_generator_lognormal_rows():
     lp = P.gen_random_lp(2, 50000, 100000, 5.12, 1.0)
     assert abs(lp.nnz - 5.12e5) / 5.12e5 < 0.03
