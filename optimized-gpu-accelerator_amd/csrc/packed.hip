@@ -892,7 +892,7 @@ PackedOutcome run_packed(const PackedData& d, const pdhg_options& o, long long m
       r.stalled = true;
       break;
     }
-    if (remaining <= handoff && remaining < K) {
+    if (remaining <= handoff && remaining < K && r.it < maxIter) {
       // A few LPs left: each continues in a single-LP run, which iterates
       // faster than the packed kernels over a handful of tasks.
       for (int q = 0; q < K; ++q) {
@@ -947,7 +947,8 @@ void write_results(const PackedData& d, const PackedOutcome& r, const std::vecto
 // their own streams.  Overwrites those slots' results; returns kernel seconds
 // and adds the iterations the runs took beyond the packed run's.
 double finish_handed(const PackedData& d, const PackedOutcome& r, const std::vector<Prepared>& P,
-                     const pdhg_options& o, Clock::time_point t0, pdhg_result* results, long long* extraIters) {
+                     const pdhg_options& o, Clock::time_point t0, pdhg_result* results, long long* extraIters,
+                     int* extraRestarts) {
   struct Slot {
     int q, b;
     std::unique_ptr<DeviceRun> run;
@@ -994,10 +995,13 @@ double finish_handed(const PackedData& d, const PackedOutcome& r, const std::vec
     sl.run->import_lanes(r.handedState[h]);
     sl.run->start(false);
     sl.own = r.handedState[h].iter[0];
+    sl.restarts = r.handedState[h].restarts[0];
     slots.push_back(std::move(sl));
   }
   double kernel = 0;
   long long extra = 0;
+  int restartsBefore = 0;
+  for (const Slot& sl : slots) restartsBefore += sl.restarts;
   bool expired = false;
   for (;;) {
     bool any = false;
@@ -1047,7 +1051,10 @@ double finish_handed(const PackedData& d, const PackedOutcome& r, const std::vec
     if (out.row_dual)
       for (int i = 0; i < p.m; ++i) out.row_dual[i] = ys[i] * p.R[i] * p.gamma;
   }
+  int restartsAfter = 0;
+  for (const Slot& sl : slots) restartsAfter += sl.restarts;
   *extraIters = extra;
+  *extraRestarts = restartsAfter - restartsBefore;
   return kernel;
 }
 
@@ -1087,13 +1094,14 @@ void solve_packed_all(const std::vector<Model>& lps, const pdhg_options& o, pdhg
   write_results(d, r, P, 0, restartBase, results);
   long long iterations = r.it, launches = r.launches, graphLaunches = r.graphLaunches;
   double kernel = r.kernel;
+  int handedRestarts = 0;
   if (!r.handed.empty()) {
     long long extra = 0;
-    kernel += finish_handed(d, r, P, o, t0, results, &extra);
+    kernel += finish_handed(d, r, P, o, t0, results, &extra, &handedRestarts);
     iterations += extra;
   }
   long long switchAt = -1;
-  int restartsTotal = 0;
+  int restartsTotal = handedRestarts;
   for (int q = 0; q < K; ++q) restartsTotal += r.restarts[q];
   if (r.stalled) {
     // The live instances continue with FP64 iterates from their FP32 point and

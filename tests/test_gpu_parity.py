@@ -384,6 +384,35 @@ def test_tiled_rows_match_one_row_per_wavefront(gpu, prec):
               f"iterations {st3['iterations']}")
 
 
+def test_packed_handoff_with_iteration_limit(gpu):
+    """The last few live LPs of a packed FP64 run move to single-LP runs with
+    their state.  With an iteration limit between the 9th and the 12th LP's
+    iteration count (not a multiple of the check interval), the handed-off
+    LPs either converge, as their one-by-one solves do, or stop at exactly
+    the limit."""
+    lps = P.gen_uneven_batch(7, 12, 400, 12000)
+    o = P.default_options(tolerance=1e-6, precision=P.FP64)
+    free, _ = P.solve_packed(lps, o)
+    its = sorted(r.iterations for r in free)
+    cap = (its[8] + its[11]) // 2 + 37
+    cap += 1 if cap % 64 == 0 else 0
+    o = P.default_options(tolerance=1e-6, precision=P.FP64, max_iterations=cap)
+    res, st = P.solve_packed(lps, o)
+    capped = 0
+    for lp, r in zip(lps, res):
+        one, _, _ = P.solve(lp, o)
+        assert r.iterations <= cap
+        if r.status == P.ITERATION_LIMIT:
+            capped += 1
+            assert r.iterations == cap
+        else:
+            assert r.status == P.OPTIMAL and one.status == P.OPTIMAL
+            assert abs(r.objective - one.objective) <= 1e-5 * (1 + abs(one.objective))
+            assert abs(r.iterations - one.iterations) <= 4 * 64
+    assert capped >= 1
+    print(f"cap {cap}: {capped} of 12 at the limit")
+
+
 _REPRO = """
 import sys, numpy as np
 sys.path.insert(0, sys.argv[1])
