@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "device.h"
+#include "tiles.h"
 
 namespace pdhgamd {
 
@@ -51,15 +52,7 @@ struct RowPlan {
   const int* ucols;
 };
 
-// Tiles: a 1024-thread workgroup (16 wavefronts, one row each) per tile and
-// the LDS it stages the tile's columns in (128 FP64 columns); two such
-// workgroups fill a CU's 32 wavefront slots and 128 of its 160 KB of LDS.
-constexpr int kTileWaves = 16;
-constexpr int kTileWG = 64 * kTileWaves;
-constexpr int kTileRows = kTileWaves;
-constexpr int kTileBytes = 64 * 1024;
-template <typename T>
-constexpr int kTileCols = kTileBytes / (64 * static_cast<int>(sizeof(T)));
+
 
 // Workgroup ids are dispatched round-robin over the 8 XCDs; this bijection
 // gives XCD x a contiguous range of logical blocks, so neighbouring rows (which
@@ -271,17 +264,6 @@ __device__ inline void tile_store(const RowPlan& p, int t, const T (&v)[kStageCo
   __syncthreads();
 }
 
-// Zero entries (column 0, weight 0) after the last nonzero of the CSR arrays
-// the iteration kernels read: whole-chunk scalar loads may run up to a
-// chunk past a row's end.
-constexpr int kCsrPad = 64;
-
-template <typename U>
-std::vector<U> csr_padded(const std::vector<U>& v) {
-  std::vector<U> out(v);
-  out.resize(v.size() + kCsrPad, U(0));
-  return out;
-}
 
 // B = 16 or 32, one row per B-lane group (G = 64 / B rows per wavefront):
 // each group's column indices and values come through the scalar cache a

@@ -616,65 +616,9 @@ struct PlanHost {
   // check kernels keep `plan`.
   bool tiled = false;
   RowPlan tplan{};
-  std::vector<int> tperm, tileRow, tileCol, ucols, loc;
+  TileSet tiles;
   int* dLoc = nullptr;
-  int tblocks = 0;
 };
-
-// Tiles of at most kTileRows rows and maxCols distinct columns over the rows
-// in locality order (by the column of their middle entry), so that rows of a
-// tile share columns; loc[e] is entry e's column numbered within its tile.
-// False when a row alone has more than maxCols entries.
-bool build_tiles(const Csr& M, int maxCols, PlanHost& h) {
-  const int rows = M.rows;
-  for (int i = 0; i < rows; ++i)
-    if (M.ptr[i + 1] - M.ptr[i] > maxCols) return false;
-  std::vector<int> key(rows);
-  for (int i = 0; i < rows; ++i) {
-    const int len = M.ptr[i + 1] - M.ptr[i];
-    key[i] = len ? M.idx[M.ptr[i] + len / 2] : 0;
-  }
-  h.tperm.resize(rows);
-  for (int i = 0; i < rows; ++i) h.tperm[i] = i;
-  std::stable_sort(h.tperm.begin(), h.tperm.end(), [&](int a, int b) { return key[a] < key[b]; });
-  h.loc.assign(static_cast<size_t>(M.nnz()) + kCsrPad, 0);
-  h.tileRow.assign(1, 0);
-  h.tileCol.assign(1, 0);
-  h.ucols.clear();
-  std::vector<int> mark(M.cols, -1), pos(M.cols, 0);
-  int tile = 0, inTile = 0;
-  for (int r = 0; r < rows; ++r) {
-    const int i = h.tperm[r];
-    for (int pass = 0; pass < 2; ++pass) {
-      const size_t keep = h.ucols.size();
-      bool fits = inTile < kTileRows;
-      for (int e = M.ptr[i]; fits && e < M.ptr[i + 1]; ++e) {
-        const int c = M.idx[e];
-        if (mark[c] == tile) continue;
-        mark[c] = tile;
-        pos[c] = static_cast<int>(h.ucols.size()) - h.tileCol.back();
-        h.ucols.push_back(c);
-        fits = pos[c] < maxCols;
-      }
-      if (fits) break;
-      // Close the tile (dropping this row's new columns) and retry in a new one.
-      for (size_t u = keep; u < h.ucols.size(); ++u) mark[h.ucols[u]] = -1;
-      h.ucols.resize(keep);
-      h.tileRow.push_back(r);
-      h.tileCol.push_back(static_cast<int>(h.ucols.size()));
-      ++tile;
-      inTile = 0;
-    }
-    for (int e = M.ptr[i]; e < M.ptr[i + 1]; ++e) h.loc[e] = pos[M.idx[e]];
-    ++inTile;
-  }
-  if (inTile > 0 || rows == 0) {
-    h.tileRow.push_back(rows);
-    h.tileCol.push_back(static_cast<int>(h.ucols.size()));
-  }
-  h.tblocks = static_cast<int>(h.tileRow.size()) - 1;
-  return h.tblocks > 0;
-}
 
 PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms, int tileCols = 0) {
   PlanHost h;
@@ -751,7 +695,7 @@ PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms, int tileCols = 0
   h.blocks = std::max(blk, 1);
   h.plan.xcd = ms.xcd ? 1 : 0;
   h.plan.perm = nullptr;
-  if (ms.tile && logB == 6 && tileCols > 0 && build_tiles(M, tileCols, h)) {
+  if (ms.tile && logB == 6 && tileCols > 0 && build_tiles(M, tileCols, h.tiles)) {
     h.tiled = true;
     h.tplan = h.plan;
     h.tplan.xcd = ms.xcd ? 1 : 0;
@@ -1359,11 +1303,11 @@ class RunImpl final : public DeviceRun {
     if (!h.perm.empty()) h.dPerm = upload_int(h.perm);
     h.plan.perm = h.dPerm;
     if (h.tiled) {
-      h.tplan.perm = upload_int(h.tperm);
-      h.tplan.tileRow = upload_int(h.tileRow);
-      h.tplan.tileCol = upload_int(h.tileCol);
-      h.tplan.ucols = upload_int(h.ucols.empty() ? std::vector<int>{0} : h.ucols);
-      h.dLoc = upload_int(h.loc);
+      h.tplan.perm = upload_int(h.tiles.perm);
+      h.tplan.tileRow = upload_int(h.tiles.tileRow);
+      h.tplan.tileCol = upload_int(h.tiles.tileCol);
+      h.tplan.ucols = upload_int(h.tiles.ucols.empty() ? std::vector<int>{0} : h.tiles.ucols);
+      h.dLoc = upload_int(h.tiles.loc);
     }
   }
   double* upload_dbl(const std::vector<double>& v) {
@@ -1424,7 +1368,7 @@ class RunImpl final : public DeviceRun {
   }
   void launch_col(bool check, int tloc) {
     if (!check && colPlan_.tiled) {
-      hipLaunchKernelGGL(k_.colTiled, dim3(colPlan_.tblocks), dim3(kTileWG), 0, stream_, colPlan_.tplan, tPtr_,
+      hipLaunchKernelGGL(k_.colTiled, dim3(colPlan_.tiles.tiles), dim3(kTileWG), 0, stream_, colPlan_.tplan, tPtr_,
                          colPlan_.dLoc, tVal_, y_, yp_, x_, xp_, x0_, c_, lc_, lo_, up_, lb_, st_, tloc, s_.rho,
                          colPart_, 0);
       return;
@@ -1436,7 +1380,7 @@ class RunImpl final : public DeviceRun {
   }
   void launch_row(bool check, int tloc) {
     if (!check && rowPlan_.tiled) {
-      hipLaunchKernelGGL(k_.rowTiled, dim3(rowPlan_.tblocks), dim3(kTileWG), 0, stream_, rowPlan_.tplan, aPtr_,
+      hipLaunchKernelGGL(k_.rowTiled, dim3(rowPlan_.tiles.tiles), dim3(kTileWG), 0, stream_, rowPlan_.tplan, aPtr_,
                          rowPlan_.dLoc, aVal_, aVal64_, xp_, y_, yp_, y0_, ax_, axp_, ax0_, rl_, ru_, rl64_, ru64_,
                          lr_, rfac_, st_, tloc, s_.rho, rowPart_, 0);
       return;

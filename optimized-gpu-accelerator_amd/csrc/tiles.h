@@ -1,0 +1,99 @@
+// Row tiles for the width-64 iteration kernels staged in LDS (host side):
+// rows in locality order, cut into tiles whose distinct columns fit the
+// workgroup's LDS, and every entry's column numbered within its tile.
+// Host-only: the kernels include it for the constants.
+#pragma once
+
+#include <algorithm>
+#include <vector>
+
+#include "core.h"
+
+namespace pdhgamd {
+
+// A 1024-thread workgroup (16 wavefronts, one row each) per tile, and the
+// LDS it stages the tile's columns in (128 FP64 columns); two such workgroups
+// fill a CU's 32 wavefront slots and 128 of its 160 KB of LDS.
+constexpr int kTileWaves = 16;
+constexpr int kTileWG = 64 * kTileWaves;
+constexpr int kTileRows = kTileWaves;
+constexpr int kTileBytes = 64 * 1024;
+template <typename T>
+constexpr int kTileCols = kTileBytes / (64 * static_cast<int>(sizeof(T)));
+
+// Zero entries (column 0, weight 0) after the last nonzero of the CSR arrays
+// the iteration kernels read: whole-chunk scalar loads may run up to a
+// chunk past a row's end.
+constexpr int kCsrPad = 64;
+
+template <typename U>
+std::vector<U> csr_padded(const std::vector<U>& v) {
+  std::vector<U> out(v);
+  out.resize(v.size() + kCsrPad, U(0));
+  return out;
+}
+
+// Tiles over the rows of a CSR matrix: tile t holds rows perm[tileRow[t] ..
+// tileRow[t+1]) and their distinct columns ucols[tileCol[t] .. tileCol[t+1]);
+// loc[e] is entry e's position among its tile's columns (padded like the CSR
+// arrays).
+struct TileSet {
+  std::vector<int> perm, tileRow, tileCol, ucols, loc;
+  int tiles = 0;
+};
+
+// Tiles of at most kTileRows rows and maxCols distinct columns over the rows
+// in locality order (by the column of their middle entry), so that rows of a
+// tile share columns.  False when a row alone has more than maxCols entries.
+inline bool build_tiles(const Csr& M, int maxCols, TileSet& t) {
+  const int rows = M.rows;
+  for (int i = 0; i < rows; ++i)
+    if (M.ptr[i + 1] - M.ptr[i] > maxCols) return false;
+  std::vector<int> key(rows);
+  for (int i = 0; i < rows; ++i) {
+    const int len = M.ptr[i + 1] - M.ptr[i];
+    key[i] = len ? M.idx[M.ptr[i] + len / 2] : 0;
+  }
+  t.perm.resize(rows);
+  for (int i = 0; i < rows; ++i) t.perm[i] = i;
+  std::stable_sort(t.perm.begin(), t.perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  t.loc.assign(static_cast<size_t>(M.nnz()) + kCsrPad, 0);
+  t.tileRow.assign(1, 0);
+  t.tileCol.assign(1, 0);
+  t.ucols.clear();
+  std::vector<int> mark(M.cols, -1), pos(M.cols, 0);
+  int tile = 0, inTile = 0;
+  for (int r = 0; r < rows; ++r) {
+    const int i = t.perm[r];
+    for (int pass = 0; pass < 2; ++pass) {
+      const size_t keep = t.ucols.size();
+      bool fits = inTile < kTileRows;
+      for (int e = M.ptr[i]; fits && e < M.ptr[i + 1]; ++e) {
+        const int c = M.idx[e];
+        if (mark[c] == tile) continue;
+        mark[c] = tile;
+        pos[c] = static_cast<int>(t.ucols.size()) - t.tileCol.back();
+        t.ucols.push_back(c);
+        fits = pos[c] < maxCols;
+      }
+      if (fits) break;
+      // Close the tile (dropping this row's new columns) and retry in a new one.
+      for (size_t u = keep; u < t.ucols.size(); ++u) mark[t.ucols[u]] = -1;
+      t.ucols.resize(keep);
+      t.tileRow.push_back(r);
+      t.tileCol.push_back(static_cast<int>(t.ucols.size()));
+      ++tile;
+      inTile = 0;
+    }
+    for (int e = M.ptr[i]; e < M.ptr[i + 1]; ++e) t.loc[e] = pos[M.idx[e]];
+    ++inTile;
+  }
+  if (inTile > 0 || rows == 0) {
+    t.tileRow.push_back(rows);
+    t.tileCol.push_back(static_cast<int>(t.ucols.size()));
+  }
+  t.tiles = static_cast<int>(t.tileRow.size()) - 1;
+  return t.tiles > 0;
+}
+
+}  // namespace pdhgamd
