@@ -4,10 +4,13 @@
 // global column indices), every instance keeps its own restart / primal
 // weight / step state, and the rows are cut into tasks: one workgroup's worth
 // of rows of one instance, grouped by row length.  The kernels are persistent
-// over a device work queue: each workgroup draws task tickets with an atomic
-// until the queue is empty, skips instances that have converged, and exits;
-// the workgroup that draws the last ticket resets the queue for the next
-// launch.  No workgroup ever waits for another, so every wave reaches its exit.
+// over eight device work queues (one per XCD): each workgroup draws task
+// tickets with an atomic until its queue is empty, skips instances that have
+// converged, and exits; the workgroup that draws a queue's last ticket resets
+// it for the next launch.  No workgroup ever waits for another, so every wave
+// reaches its exit.  When few live tasks remain they run as direct launches
+// (one workgroup per task), and the last few live LPs finish in single-LP
+// runs of the batched engine.
 //
 // The reference solves such LPs one at a time (solvePdhg, src/pdhg/pdhg.h:73);
 // the per-instance algorithm is the same (src/pdhg/pdhg.cpp:483-687).

@@ -7,15 +7,19 @@
 // (a power of two) is chosen per row-length bucket so that each lane streams a
 // few consecutive nonzeros.  Rows of one bucket are packed into 256-thread
 // workgroups; a workgroup only ever holds rows of one bucket, so shuffle
-// reductions are uniform inside it.  No atomics anywhere: every cross-lane or
-// cross-block sum is a fixed-order tree.
+// reductions are uniform inside it.  At B = 64 a wavefront is one row (lane =
+// instance); the iterations without checks may instead run over row tiles
+// whose distinct gathered rows a 1024-thread workgroup stages in LDS once
+// (k_col_tiled / k_row_tiled, same sums).  No atomics anywhere: every
+// cross-lane or cross-block sum is a fixed-order tree.
 //
 // One PDHG iteration is two kernels (reference src/pdhg/pdhg.cpp:588-662):
 //   k_col : aty = A^T y, x+ = proj(x - tau (c - aty)), Halpern update of x
 //   k_row : ax+ = A x+, reflected dual step, Halpern update of y and ax
 // and a check iteration adds k_colcheck (FP64 A^T y+ and the column KKT terms)
 // and k_finalize (FP64 KKT / gap / restart / primal weight, one block per
-// instance).  Restarts are applied by the first iteration after the check.
+// instance, at each instance's own check points).  Restarts are applied by
+// the first iteration after the check.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
