@@ -414,9 +414,12 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
           r.iterations = doneIter[b];
         } else {
           PDHG_REQUIRE(final || limitHit[b], "internal: extracting a live instance");
-          r.status = expired ? PDHG_TIME_LIMIT : PDHG_ITERATION_LIMIT;
+          // A lane stopped at its own limit stays so even if the run's time
+          // limit expires before it is extracted.
+          const bool timedOut = expired && !limitHit[b];
+          r.status = timedOut ? PDHG_TIME_LIMIT : PDHG_ITERATION_LIMIT;
           const long long own = limitHit[b] ? o.max_iterations : it - loadIt[b];
-          r.iterations = expired ? own - 1 : own;
+          r.iterations = timedOut ? own - 1 : own;
         }
         r.restarts = restarts[b];
         r.objective = cr.pobj;

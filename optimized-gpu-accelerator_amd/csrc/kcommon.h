@@ -23,7 +23,6 @@ namespace pdhgamd {
 
 namespace {
 
-
 #ifndef PDHG_LOG_WG
 #define PDHG_LOG_WG 8
 #endif
@@ -51,8 +50,6 @@ struct RowPlan {
   const int* tileCol;
   const int* ucols;
 };
-
-
 
 // Workgroup ids are dispatched round-robin over the 8 XCDs; this bijection
 // gives XCD x a contiguous range of logical blocks, so neighbouring rows (which
@@ -263,7 +260,6 @@ __device__ inline void tile_store(const RowPlan& p, int t, const T (&v)[kStageCo
   }
   __syncthreads();
 }
-
 
 // B = 16 or 32, one row per B-lane group (G = 64 / B rows per wavefront):
 // each group's column indices and values come through the scalar cache a
@@ -480,7 +476,6 @@ template <bool SPLIT, int LOGB, typename T>
 __device__ inline T cld(const T* __restrict__ p, const Ld& l, int i, int b) {
   return l.mask ? vld<SPLIT, LOGB>(p, i, b) : p[i];
 }
-
 
 // Check logic of one instance from its nine fixed-order sums S (column side:
 // ||x - x+||^2, ||x+ - x0||^2, dual residual^2, c'x+, column dual objective;
