@@ -384,6 +384,25 @@ def test_tiled_rows_match_one_row_per_wavefront(gpu, prec):
               f"iterations {st3['iterations']}")
 
 
+def test_streamed_family_time_limit(gpu):
+    """A run's time limit (a host-side clock, the reference's Deadline,
+    src/util/timer.h:22-33) stops a continuous batch: converged cases keep
+    their answers, running ones report TIME_LIMIT with the iterations they
+    ran, and cases never started report TIME_LIMIT with none."""
+    lp = P.gen_refinery(3)
+    _, costs = P.gen_family(lp, 4, 96, 0.3)
+    o = P.default_options(tolerance=1e-6, precision=P.FP64, time_limit=0.3)
+    res, st, _ = P.solve_batch(lp, costs, o)
+    status = [r.status for r in res]
+    assert all(s in (P.OPTIMAL, P.TIME_LIMIT) for s in status)
+    assert status.count(P.TIME_LIMIT) >= 32  # the 32 cases past the first 64 never ran
+    never = [r for r in res if r.status == P.TIME_LIMIT and r.iterations == 0]
+    running = [r for r in res if r.status == P.TIME_LIMIT and r.iterations > 0]
+    assert len(never) >= 32 and len(running) >= 1
+    assert all(r.iterations < 1_000_000 for r in res)
+    print(f"{len(running)} stopped while running, {len(never)} never started, {status.count(P.OPTIMAL)} converged")
+
+
 def test_packed_handoff_with_iteration_limit(gpu):
     """The last few live LPs of a packed FP64 run move to single-LP runs with
     their state.  With an iteration limit between the 9th and the 12th LP's
