@@ -945,13 +945,15 @@ void write_results(const PackedData& d, const PackedOutcome& r, const std::vecto
 }
 
 // The handed-off slots of a packed run, each continued in its own single-LP
-// run (FP64 iterates, one lane) from the exported state, with the same
-// options and its own iteration limit; the runs' blocks are interleaved on
-// their own streams.  Overwrites those slots' results; returns kernel seconds
-// and adds the iterations the runs took beyond the packed run's.
+// run (FP64 iterates, one lane) from the exported state, with the packed
+// run's iteration limit maxIter on its own count; the runs' blocks are
+// interleaved on their own streams.  Overwrites those slots' results
+// (iterations and restarts offset by itBase / restartBase, as the packed
+// run's); returns kernel seconds and adds the iterations the runs took beyond
+// the packed run's.
 double finish_handed(const PackedData& d, const PackedOutcome& r, const std::vector<Prepared>& P,
-                     const pdhg_options& o, Clock::time_point t0, pdhg_result* results, long long* extraIters,
-                     int* extraRestarts) {
+                     const pdhg_options& o, long long maxIter, long long itBase, const std::vector<int>& restartBase,
+                     Clock::time_point t0, pdhg_result* results, long long* extraIters, int* extraRestarts) {
   struct Slot {
     int q, b;
     std::unique_ptr<DeviceRun> run;
@@ -992,7 +994,7 @@ double finish_handed(const PackedData& d, const PackedOutcome& r, const std::vec
     sc.gamma = d.gamma[q];
     sc.offset = d.offset[q];
     sc.tol = o.tolerance;
-    sc.maxIter = o.max_iterations;
+    sc.maxIter = maxIter;
     const int live = 0;
     sl.run->init_state(sc, &d.omega[q], &d.cNorm[q], &d.rhsNorm[q], &live);
     sl.run->import_lanes(r.handedState[h]);
@@ -1010,7 +1012,7 @@ double finish_handed(const PackedData& d, const PackedOutcome& r, const std::vec
     bool any = false;
     for (Slot& sl : slots) {
       if (sl.finished) continue;
-      const long long left = o.max_iterations - sl.own;
+      const long long left = maxIter - sl.own;
       const int nIt = static_cast<int>(std::min<long long>(o.check_every - sl.own % o.check_every, left));
       sl.run->run_block(nIt);
       sl.own += nIt;
@@ -1020,7 +1022,7 @@ double finish_handed(const PackedData& d, const PackedOutcome& r, const std::vec
     for (Slot& sl : slots) {
       if (sl.finished) continue;
       sl.run->fetch(&sl.rec, &sl.done, &sl.doneIter, &sl.restarts);
-      if (sl.done || sl.own >= o.max_iterations) {
+      if (sl.done || sl.own >= maxIter) {
         sl.finished = true;
         sl.limit = !sl.done;
       }
@@ -1036,8 +1038,8 @@ double finish_handed(const PackedData& d, const PackedOutcome& r, const std::vec
     pdhg_result& out = results[sl.b];
     const Prepared& p = P[sl.b];
     out.status = sl.done ? PDHG_OPTIMAL : (expired && !sl.limit ? PDHG_TIME_LIMIT : PDHG_ITERATION_LIMIT);
-    out.iterations = sl.done ? sl.doneIter : sl.own;
-    out.restarts = sl.restarts;
+    out.iterations = itBase + (sl.done ? sl.doneIter : sl.own);
+    out.restarts = restartBase[sl.b] + sl.restarts;
     const CheckRecord& c = sl.rec;
     out.objective = c.pobj;
     out.dual_objective = c.dobj;
@@ -1100,7 +1102,7 @@ void solve_packed_all(const std::vector<Model>& lps, const pdhg_options& o, pdhg
   int handedRestarts = 0;
   if (!r.handed.empty()) {
     long long extra = 0;
-    kernel += finish_handed(d, r, P, o, t0, results, &extra, &handedRestarts);
+    kernel += finish_handed(d, r, P, o, o.max_iterations, 0, restartBase, t0, results, &extra, &handedRestarts);
     iterations += extra;
   }
   long long switchAt = -1;
@@ -1126,9 +1128,16 @@ void solve_packed_all(const std::vector<Model>& lps, const pdhg_options& o, pdhg
       restartBase[b] = r.restarts[q];
     }
     const PackedData d2 = pack(lps, P, live, &xw, &yw, &ww);
-    const PackedOutcome r2 = run_packed<double>(d2, o, o.max_iterations - r.it, t0, false, 0);
+    const PackedOutcome r2 = run_packed<double>(d2, o, o.max_iterations - r.it, t0, false, kHandoff);
     write_results(d2, r2, P, r.it, restartBase, results);
     iterations += r2.it;
+    if (!r2.handed.empty()) {
+      long long extra = 0;
+      int more = 0;
+      kernel += finish_handed(d2, r2, P, o, o.max_iterations - r.it, r.it, restartBase, t0, results, &extra, &more);
+      iterations += extra;
+      restartsTotal += more;
+    }
     launches += r2.launches;
     graphLaunches += r2.graphLaunches;
     kernel += r2.kernel;
