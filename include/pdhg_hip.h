@@ -96,7 +96,7 @@ typedef struct pdhg_options {
   int rescale;                 /* objective/bound rescaling beta, gamma (1) */
   double reflection;           /* rho (1.0) */
   int log_interval;            /* log every N checks to stderr (0 = off) */
-  int precision;               /* PDHG_FP32 (default), PDHG_FP64 or PDHG_FP32_AUTO */
+  int precision;               /* PDHG_FP64 (default), PDHG_FP32 or PDHG_FP32_AUTO */
   int use_graph;               /* capture check_every iterations in a hipGraph (1) */
   int lanes_per_row;           /* 0 = autotune; else nnz-per-lane target (1..64) */
   int row_map;                 /* 0 = auto, 1 = row-length buckets, 2 = one mapping,

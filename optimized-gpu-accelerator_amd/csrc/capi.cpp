@@ -74,7 +74,7 @@ void pdhg_default_options(pdhg_options* o) {
   o->rescale = 1;
   o->reflection = 1.0;
   o->log_interval = 0;
-  o->precision = PDHG_FP32;
+  o->precision = PDHG_FP64;  // the reference's arithmetic; FP32 iterates are opt-in
   o->use_graph = 1;
   o->lanes_per_row = 0;
   o->row_map = 0;

@@ -738,8 +738,8 @@ const DeviceInfo& device_info(int device) {
     d.reason = "hipGetDeviceProperties failed";
     return d;
   }
-  d.name = p.name;
   d.arch = p.gcnArchName;
+  d.name = p.name[0] ? std::string(p.name) : "AMD " + d.arch.substr(0, d.arch.find(':'));
   d.computeUnits = p.multiProcessorCount;
   d.memoryBytes = p.totalGlobalMem;
   // memoryClockRate (kHz) x bus width x 2 is the DDR formula; it reads half
