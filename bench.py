@@ -197,7 +197,7 @@ def calibrate(threads, out_path):
     out = {"gpu": P.gpu_info(), "cpu_threads": threads, "host_nproc": os.cpu_count(),
            "iterations_per_point": RF.ITERATIONS, "pdhg_iteration_timing": rows,
            "fp64": RF.refit(rows, "gpu_fp64"), "fp32": RF.refit(rows, "gpu_fp32")}
-    out["router_model"] = {"weights": out["fp32"]["weights"]}  # the engine's default precision
+    out["router_model"] = {"weights": out["fp64"]["weights"]}  # the engine's default precision (FP64)
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1)
     log(f"crossover nnz (measured): fp64 {out['fp64']['crossover_nnz_measured']}, "
