@@ -13,6 +13,7 @@ import scipy.sparse as sp
 import oracle_lib as O
 import pdhg_amd as P
 from conftest import load_netlib
+from lp_cases import degenerate_costs, degenerate_lp, infeasible_lp
 
 pytestmark = pytest.mark.gpu
 
@@ -491,3 +492,60 @@ def test_streamed_family_iteration_limit(gpu):
             assert r.iterations == cap, q
         assert r.objective == ref.objective and r.relative_kkt == ref.relative_kkt, q
         assert np.array_equal(r.x, ref.x) and np.array_equal(r.y, ref.y), q
+
+
+def test_degenerate_structure_matches_oracle(gpu):
+    """Empty, free and ranged rows, empty, free and fixed columns (lp_cases):
+    an FP64 solve against the oracle and the closed-form optimum, and a
+    width-64 FP64 batch whose LDS-staged tiles hold the empty rows and
+    columns, case by case against the oracle's batch and bit-equal to one
+    row per wavefront."""
+    lp, obj = degenerate_lp()
+    ref, _, _, _ = O.oracle_solve(lp, tol=1e-6)
+    r, _, _ = P.solve(lp, P.default_options(tolerance=1e-6, precision=P.FP64))
+    print(f"single: {r.iterations} iterations (oracle {ref[0]['iterations']}), objective {r.objective:.10g} "
+          f"(oracle {ref[0]['objective']:.10g}, closed form {obj:.10g})")
+    assert r.status == P.OPTIMAL and ref[0]["status"] == P.OPTIMAL
+    assert abs(r.iterations - ref[0]["iterations"]) <= 4 * 64
+    assert abs(r.objective - ref[0]["objective"]) <= 1e-6 * (1 + abs(obj))
+    assert abs(r.objective - obj) <= 1e-5 * (1 + abs(obj))
+    relP, relD, gap, _ = host_kkt(lp, r.x, r.y)
+    assert max(relP, relD, gap) <= 1.5e-6, (relP, relD, gap)
+    assert np.allclose(r.x[-3:], [0.0, 0.0, 2.0], rtol=0, atol=1e-12)
+    assert np.all(np.abs(r.y[-2:]) <= 1e-12)  # the empty free row and the free row
+
+    costs = degenerate_costs(lp, 64)
+    ref_b, _, _, _ = O.oracle_solve(lp, costs, tol=1e-6)
+    runs = {}
+    for row_map in (1, 3):
+        o = P.default_options(tolerance=1e-6, precision=P.FP64, row_map=row_map, xcd_remap=1, compact=0)
+        runs[row_map] = P.solve_batch(lp, costs, o)
+    (plain, st1, _), (tiled, st3, _) = runs[1], runs[3]
+    assert (st1["col_row_map"], st1["row_row_map"]) == (1, 1)
+    assert (st3["col_row_map"], st3["row_row_map"]) == (3, 3)
+    for q in range(64):
+        a, b, o = tiled[q], plain[q], ref_b[q]
+        assert a.iterations == b.iterations and a.objective == b.objective, q
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y), q
+        assert a.status == P.OPTIMAL and o["status"] == P.OPTIMAL, q
+        assert abs(a.iterations - o["iterations"]) <= 4 * 64 + 0.02 * o["iterations"], q
+        assert abs(a.objective - o["objective"]) <= 1e-6 * (1 + abs(o["objective"])), q
+    print(f"batch: iterations {min(r.iterations for r in tiled)}..{max(r.iterations for r in tiled)}")
+
+
+@pytest.mark.parametrize("k", [1, 64])
+def test_infeasible_lp_stops_at_iteration_limit(gpu, k):
+    """PDHG has no infeasibility detection (src/pdhg/pdhg.cpp:593-661): on
+    x1 + x2 <= -1, x >= 0 the solve runs to its limit and reports
+    ITERATION_LIMIT with exactly that many iterations, as the oracle does."""
+    lp = infeasible_lp()
+    ref, _, _, _ = O.oracle_solve(lp, tol=1e-6, max_iter=3000)
+    assert (ref[0]["status"], ref[0]["iterations"]) == (P.ITERATION_LIMIT, 3000)
+    o = P.default_options(tolerance=1e-6, precision=P.FP64, max_iterations=3000)
+    if k == 1:
+        res = [P.solve(lp, o)[0]]
+    else:
+        res = P.solve_batch(lp, np.repeat(lp.c[None, :], k, axis=0), o)[0]
+    for r in res:
+        assert (r.status, r.iterations) == (P.ITERATION_LIMIT, 3000)
+        assert r.relative_kkt > 1e-3

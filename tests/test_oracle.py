@@ -14,6 +14,7 @@ import pytest
 import oracle_lib as O
 import pdhg_amd as P
 from conftest import load_netlib, load_plan
+from lp_cases import degenerate_lp, infeasible_lp
 
 needs_ref = pytest.mark.skipif(not O.ref_available(), reason="compiled reference (oracle/_ref) not built")
 
@@ -55,6 +56,35 @@ def test_oracle_bitwise_equals_reference_netlib(name):
     r, xr, yr = O.ref_solve(lp, tol=1e-6, max_iter=200_000)
     assert (o[0]["iterations"], o[0]["status"], o[0]["objective"]) == (r["iterations"], r["status"], r["objective"])
     assert np.array_equal(xo[0], xr) and np.array_equal(yo[0], yr)
+
+
+@needs_ref
+def test_oracle_bitwise_equals_reference_degenerate_structure():
+    """Empty, free and ranged rows, empty, free and fixed columns."""
+    lp, _ = degenerate_lp()
+    o, xo, yo, _ = O.oracle_solve(lp, tol=1e-6)
+    r, xr, yr = O.ref_solve(lp, tol=1e-6)
+    assert (o[0]["iterations"], o[0]["status"], o[0]["objective"]) == (r["iterations"], r["status"], r["objective"])
+    assert o[0]["spectral_norm"] == r["spectral_norm"]
+    assert np.array_equal(xo[0], xr) and np.array_equal(yo[0], yr)
+
+
+@needs_ref
+def test_oracle_iteration_limit_equals_reference_on_infeasible_lp():
+    lp = infeasible_lp()
+    o, xo, yo, _ = O.oracle_solve(lp, tol=1e-6, max_iter=3000)
+    r, xr, yr = O.ref_solve(lp, tol=1e-6, max_iter=3000)
+    assert (o[0]["iterations"], o[0]["status"]) == (r["iterations"], r["status"]) == (3000, P.ITERATION_LIMIT)
+    assert np.array_equal(xo[0], xr) and np.array_equal(yo[0], yr)
+
+
+def test_oracle_degenerate_structure_reaches_closed_form():
+    lp, obj = degenerate_lp()
+    o, xs, ys, _ = O.oracle_solve(lp, tol=1e-6)
+    assert o[0]["status"] == P.OPTIMAL and o[0]["relative_kkt"] <= 1e-6
+    assert abs(o[0]["objective"] - obj) <= 1e-5 * (1 + abs(obj))
+    # the added columns at their optimal values: lower bound, initial point, fixed value
+    assert xs[0][-3:].tolist() == [0.0, 0.0, 2.0]
 
 
 @needs_ref
