@@ -549,3 +549,27 @@ def test_infeasible_lp_stops_at_iteration_limit(gpu, k):
     for r in res:
         assert (r.status, r.iterations) == (P.ITERATION_LIMIT, 3000)
         assert r.relative_kkt > 1e-3
+
+
+def test_packed_edge_cases_match_oracle(gpu):
+    """The packed kernels on the edge-case LPs: two degenerate-structure LPs,
+    an infeasible LP and two ordinary ones in one FP64 run with an iteration
+    limit past the degenerate LPs' own counts.  The solvable LPs agree with
+    the oracle; the infeasible one stops at exactly the limit."""
+    cap = 30001
+    deg = [degenerate_lp(seed=5), degenerate_lp(seed=6, m=150, n=320)]
+    plain = P.gen_uneven_batch(7, 2, 400, 3000)
+    lps = [deg[0][0], infeasible_lp(), plain[0], deg[1][0], plain[1]]
+    res, st = P.solve_packed(lps, P.default_options(tolerance=1e-6, precision=P.FP64, max_iterations=cap))
+    assert st["batch"] == len(lps)
+    assert (res[1].status, res[1].iterations) == (P.ITERATION_LIMIT, cap)
+    for q in (0, 2, 3, 4):
+        o, _, _, _ = O.oracle_solve(lps[q], tol=1e-6, max_iter=cap)
+        r = res[q]
+        print(f"LP {q}: {r.iterations} iterations (oracle {o[0]['iterations']}), objective {r.objective:.10g} "
+              f"(oracle {o[0]['objective']:.10g})")
+        assert r.status == P.OPTIMAL and o[0]["status"] == P.OPTIMAL, q
+        assert abs(r.iterations - o[0]["iterations"]) <= 4 * 64 + 0.02 * o[0]["iterations"], q
+        assert abs(r.objective - o[0]["objective"]) <= 1e-6 * (1 + abs(o[0]["objective"])), q
+    for q, (lp, obj) in zip((0, 3), deg):
+        assert abs(res[q].objective - obj) <= 1e-5 * (1 + abs(obj))
