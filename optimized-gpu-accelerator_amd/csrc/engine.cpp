@@ -36,6 +36,9 @@ bool same_pointer(const std::vector<CaseData>& cs, const double* CaseData::*f) {
 
 Model model_from(const pdhg_lp* lp) {
   PDHG_REQUIRE(lp && lp->m >= 0 && lp->n >= 0, "bad LP");
+  PDHG_REQUIRE(lp->col_start, "null column pointers");
+  PDHG_REQUIRE(lp->n == 0 || (lp->cost && lp->col_lower && lp->col_upper), "null column data");
+  PDHG_REQUIRE(lp->m == 0 || (lp->row_lower && lp->row_upper), "null row bounds");
   Model md;
   md.m = lp->m;
   md.n = lp->n;
@@ -44,6 +47,7 @@ Model model_from(const pdhg_lp* lp) {
   md.A.start.assign(lp->col_start, lp->col_start + lp->n + 1);
   const int nnz = md.A.start.back();
   PDHG_REQUIRE(md.A.start[0] == 0 && nnz >= 0, "bad column pointers");
+  PDHG_REQUIRE(nnz == 0 || (lp->row_index && lp->value), "null matrix arrays");
   for (int j = 0; j < lp->n; ++j) PDHG_REQUIRE(md.A.start[j] <= md.A.start[j + 1], "column pointers not monotone");
   md.A.index.assign(lp->row_index, lp->row_index + nnz);
   md.A.value.assign(lp->value, lp->value + nnz);
@@ -54,6 +58,8 @@ Model model_from(const pdhg_lp* lp) {
   md.c.assign(lp->cost, lp->cost + lp->n);
   md.l.assign(lp->col_lower, lp->col_lower + lp->n);
   md.u.assign(lp->col_upper, lp->col_upper + lp->n);
+  for (int j = 0; j < lp->n; ++j)
+    PDHG_REQUIRE(!std::isnan(md.c[j]) && !std::isnan(md.l[j]) && !std::isnan(md.u[j]), "NaN in column data");
   md.L.assign(lp->row_lower, lp->row_lower + lp->m);
   md.U.assign(lp->row_upper, lp->row_upper + lp->m);
   md.offset = lp->obj_offset;
@@ -145,14 +151,25 @@ std::vector<MapSetup> map_candidates(int B) {
 
 }  // namespace
 
+void check_options(const pdhg_options& o) {
+  PDHG_REQUIRE(o.use_gpu, "use_gpu = 0: this engine has no CPU backend");
+  PDHG_REQUIRE(o.check_every >= 2, "check_every must be >= 2");
+  PDHG_REQUIRE(o.max_iterations >= 1, "max_iterations must be >= 1");
+  PDHG_REQUIRE(!std::isnan(o.tolerance), "tolerance is NaN");
+  PDHG_REQUIRE(!std::isnan(o.time_limit), "time_limit is NaN");
+  PDHG_REQUIRE(o.precision == PDHG_FP32 || o.precision == PDHG_FP64 || o.precision == PDHG_FP32_AUTO,
+               "unknown precision");
+  PDHG_REQUIRE(o.lanes_per_row >= 0 && o.lanes_per_row <= 64, "lanes_per_row must be in 0..64");
+  PDHG_REQUIRE(o.row_map >= 0 && o.row_map <= 3, "row_map must be in 0..3");
+  PDHG_REQUIRE(o.xcd_remap >= -1 && o.xcd_remap <= 1, "xcd_remap must be -1, 0 or 1");
+}
+
 void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pdhg_options& o, pdhg_result* results,
                  pdhg_stats* stats) {
   const auto tSetup = Clock::now();
   const int k = static_cast<int>(cases.size());
   PDHG_REQUIRE(k >= 1, "no cases");
-  PDHG_REQUIRE(o.use_gpu, "use_gpu = 0: this engine has no CPU backend");
-  PDHG_REQUIRE(o.check_every >= 2, "check_every must be >= 2");
-  PDHG_REQUIRE(o.max_iterations >= 1, "max_iterations must be >= 1");
+  check_options(o);
   const DeviceInfo& di = device_info(o.device);
   if (!di.available) throw Error("GPU unavailable: " + di.reason);
   const int n = base.n, m = base.m;

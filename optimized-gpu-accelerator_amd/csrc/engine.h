@@ -22,7 +22,13 @@ struct CaseData {
 void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pdhg_options& o, pdhg_result* results,
                  pdhg_stats* stats);
 
+// A copy of a C-ABI LP, checked as the reference's Model::validate does
+// (src/core/model.cpp:103-116): no NaN in column data, finite coefficients;
+// plus the checks of the ABI itself (non-NULL arrays, index ranges).
 Model model_from(const pdhg_lp* lp);
+
+// Rejects options outside their documented ranges, before any device work.
+void check_options(const pdhg_options& o);
 
 // Unrelated LPs advanced together by the persistent packed kernels.
 void solve_packed(const std::vector<Model>& lps, const pdhg_options& o, pdhg_result* results, pdhg_stats* stats);

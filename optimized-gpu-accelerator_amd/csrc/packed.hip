@@ -1165,10 +1165,7 @@ void solve_packed_all(const std::vector<Model>& lps, const pdhg_options& o, pdhg
 
 void solve_packed(const std::vector<Model>& lps, const pdhg_options& o, pdhg_result* results, pdhg_stats* stats) {
   PDHG_REQUIRE(!lps.empty(), "no LPs");
-  PDHG_REQUIRE(o.use_gpu, "use_gpu = 0: this engine has no CPU backend");
-  PDHG_REQUIRE(o.check_every >= 2 && o.max_iterations >= 1, "bad iteration options");
-  PDHG_REQUIRE(o.precision == PDHG_FP32 || o.precision == PDHG_FP64 || o.precision == PDHG_FP32_AUTO,
-               "unknown precision");
+  check_options(o);
   const DeviceInfo& di = device_info(o.device);
   if (!di.available) throw Error("GPU unavailable: " + di.reason);
   solve_packed_all(lps, o, results, stats);

@@ -1,6 +1,7 @@
 """CPU-side tests of the product library: the C-ABI surface, the host set-up
 (bit-identical to the reference), the certifier and the generators.  No GPU
 compute is called here."""
+import ctypes as C
 import os
 import re
 
@@ -45,6 +46,36 @@ def test_bad_input_is_rejected():
     bad.index[3] = lp.m + 5
     with pytest.raises(RuntimeError, match="row index out of range"):
         P.safe_dual_bound(bad, np.zeros(lp.m))
+
+
+def test_bad_solver_input_is_rejected_before_device_work():
+    """NaN column data is rejected as the reference's Model::validate does
+    (src/core/model.cpp:111-113), NULL arrays and out-of-range options by the
+    ABI's own checks; all before any device is touched, so the messages are
+    the same with or without a GPU."""
+    lp = P.gen_random_lp(1, 50, 100)
+    for field in ("c", "l", "u"):
+        v = getattr(lp, field).copy()
+        v[7] = np.nan
+        bad = P.LP(lp.m, lp.n, lp.start, lp.index, lp.value, lp.c, lp.l, lp.u, lp.L, lp.U)
+        setattr(bad, field, v)
+        with pytest.raises(RuntimeError, match="NaN in column data"):
+            P.solve(bad)
+        with pytest.raises(RuntimeError, match="NaN in column data"):
+            P.solve_packed([lp, bad])
+    s = lp.struct()
+    s.col_start = None
+    res = P.ResultStruct()
+    assert P.lib().pdhg_solve(C.byref(s), None, C.byref(res), None) == -1
+    assert b"null column pointers" in P.lib().pdhg_last_error()
+    for kw, msg in ((dict(precision=7), "unknown precision"), (dict(row_map=4), "row_map"),
+                    (dict(lanes_per_row=65), "lanes_per_row"), (dict(xcd_remap=2), "xcd_remap"),
+                    (dict(tolerance=float("nan")), "tolerance is NaN"), (dict(check_every=1), "check_every"),
+                    (dict(max_iterations=0), "max_iterations")):
+        with pytest.raises(RuntimeError, match=msg):
+            P.solve(lp, P.default_options(**kw))
+        with pytest.raises(RuntimeError, match=msg):
+            P.solve_packed([lp], P.default_options(**kw))
 
 
 @pytest.mark.parametrize("gen", ["c0", "c1", "netlib", "refinery"])
