@@ -131,6 +131,36 @@ def pcie_rates(size=256 << 20):
     return rates[0], rates[1]
 
 
+def spmv_rates(lp, reps=200):
+    """Standalone CSR SpMV on the scaled matrix: the op-level backend's k_spmv
+    (no atomics, rows bucketed by length; the reference's Backend::spmv and
+    spmvT, src/pdhg/pdhg.cpp:93-103), A x and A^T y at widths 1 and 64, FP64
+    and FP32 values, HIP events over `reps` launches on the backend's stream.
+    Algorithmic bytes: values and indices once, row pointers, input and output
+    vectors once."""
+    import scipy.sparse as sp
+    pr = P.prepare(lp)
+    A = sp.csr_matrix((pr["val"], pr["idx"], pr["ptr"]), shape=(lp.m, lp.n))
+    AT = A.T.tocsr()
+    AT.sort_indices()
+    rfac, cfac = pr["beta"] / pr["R"], pr["gamma"] / pr["C"]
+    rng = np.random.default_rng(0)
+    out = {}
+    for prec, name, vs in ((P.FP64, "fp64", 8), (P.FP32, "fp32", 4)):
+        for k in (1, 64):
+            be = P.Backend(A, AT, rfac, cfac, k=k, precision=prec)
+            be.set(P.V_X, rng.standard_normal((lp.n, k)))
+            be.set(P.V_Y, rng.standard_normal((lp.m, k)))
+            r = {}
+            for transpose, key, rows, cols in ((False, "Ax", lp.m, lp.n), (True, "ATy", lp.n, lp.m)):
+                ms = be.time_spmv(transpose, reps)
+                nbytes = lp.nnz * (vs + 4) + (rows + 1) * 4 + (rows + cols) * k * vs
+                r[key] = {"us": ms * 1e3, "GBs": nbytes / (ms * 1e-3) / 1e9, "bytes": nbytes}
+            be.close()
+            out[f"{name}_B{k}"] = r
+    return out
+
+
 def cpu_baseline(lp, costs, gpu_iters, seconds, threads):
     """The reference CPU PDHG (its own sources, 16 threads) on the same batched
     family, run for a bounded number of iterations; solves/s extrapolated to the
@@ -361,6 +391,10 @@ def main():
     log(f"kernels: k_col {kern['k_col'][1] * 1e3:.1f} us ({kern['k_col'][0] / kern['k_col'][1] / 1e6:.0f} GB/s), "
         f"k_row {kern['k_row'][1] * 1e3:.1f} us ({kern['k_row'][0] / kern['k_row'][1] / 1e6:.0f} GB/s)")
 
+    spmv = spmv_rates(lp)
+    log("spmv (op-level k_spmv, scaled refinery matrix): " + ", ".join(
+        f"{k} A x {v['Ax']['GBs']:.0f} / A^T y {v['ATy']['GBs']:.0f} GB/s" for k, v in spmv.items()))
+
     # Plain single-instance GPU PDHG (cold start) on a sample of the same instances.
     tp = time.perf_counter()
     plain_ok = 0
@@ -403,6 +437,7 @@ def main():
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "algorithmic_bytes": dbytes,
                      "launch_us": dms * 1e3},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "spmv": spmv,
         "value_with_pcie": certified / elapsed,
         "pcie": {"seconds": pcie_s, "bytes_up": up_bytes, "bytes_down": down_bytes,
                  "gb_per_s_up": up_bw / 1e9, "gb_per_s_down": down_bw / 1e9},
