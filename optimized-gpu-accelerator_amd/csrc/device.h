@@ -18,8 +18,13 @@ struct DeviceInfo {
   int ordinal = 0;
 };
 
-// Probes the HIP runtime once per process (the analogue of the reference's
-// CudaContext singleton, src/gpu/cuda_driver.cpp:148-162).
+// Device ordinals the engine addresses (a fixed table: the probe results keep
+// their addresses for the life of the process).
+constexpr int kMaxDevices = 64;
+
+// Probes the HIP runtime once per device and process (the analogue of the
+// reference's CudaContext singleton, src/gpu/cuda_driver.cpp:148-162).  An
+// ordinal outside 0 .. kMaxDevices-1 reads as an unavailable device.
 const DeviceInfo& device_info(int device);
 
 // Row-to-lane mapping of one CSR matrix.

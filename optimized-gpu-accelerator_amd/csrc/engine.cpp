@@ -162,6 +162,7 @@ void check_options(const pdhg_options& o) {
   PDHG_REQUIRE(o.lanes_per_row >= 0 && o.lanes_per_row <= 64, "lanes_per_row must be in 0..64");
   PDHG_REQUIRE(o.row_map >= 0 && o.row_map <= 3, "row_map must be in 0..3");
   PDHG_REQUIRE(o.xcd_remap >= -1 && o.xcd_remap <= 1, "xcd_remap must be -1, 0 or 1");
+  PDHG_REQUIRE(o.device >= 0 && o.device < kMaxDevices, "device ordinal out of range");
 }
 
 void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pdhg_options& o, pdhg_result* results,

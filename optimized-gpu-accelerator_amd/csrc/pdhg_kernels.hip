@@ -715,14 +715,16 @@ std::mutex gInfoMutex;
 // Device probe (gpuAvailable / gpuInfoJson)
 // ---------------------------------------------------------------------------
 const DeviceInfo& device_info(int device) {
-  static std::vector<DeviceInfo> cache;
-  static std::vector<char> probed;
+  static DeviceInfo cache[kMaxDevices];
+  static bool probed[kMaxDevices] = {};
+  static const DeviceInfo outOfRange = [] {
+    DeviceInfo d;
+    d.ordinal = -1;
+    d.reason = "device ordinal outside 0.." + std::to_string(kMaxDevices - 1);
+    return d;
+  }();
+  if (device < 0 || device >= kMaxDevices) return outOfRange;
   std::lock_guard<std::mutex> lk(gInfoMutex);
-  if (device < 0) device = 0;
-  if (static_cast<int>(cache.size()) <= device) {
-    cache.resize(device + 1);
-    probed.resize(device + 1, 0);
-  }
   DeviceInfo& d = cache[device];
   if (probed[device]) return d;
   probed[device] = 1;

@@ -71,7 +71,8 @@ def test_bad_solver_input_is_rejected_before_device_work():
     for kw, msg in ((dict(precision=7), "unknown precision"), (dict(row_map=4), "row_map"),
                     (dict(lanes_per_row=65), "lanes_per_row"), (dict(xcd_remap=2), "xcd_remap"),
                     (dict(tolerance=float("nan")), "tolerance is NaN"), (dict(check_every=1), "check_every"),
-                    (dict(max_iterations=0), "max_iterations")):
+                    (dict(max_iterations=0), "max_iterations"), (dict(device=-1), "device ordinal"),
+                    (dict(device=1 << 30), "device ordinal")):
         with pytest.raises(RuntimeError, match=msg):
             P.solve(lp, P.default_options(**kw))
         with pytest.raises(RuntimeError, match=msg):
