@@ -225,7 +225,9 @@ int pdhg_certify_batch(const pdhg_lp* base, const pdhg_case* cases, int k, const
       pdhgamd::implied_bounds(md.A, pick(0, &pdhg_case::row_lower, md.L), pick(0, &pdhg_case::row_upper, md.U), lo0,
                               up0);
     }
-    const int nt = std::max(1, std::min(threads, k));
+    // At most one thread per case and per hardware thread of the host.
+    const int hw = std::max(1, static_cast<int>(std::thread::hardware_concurrency()));
+    const int nt = std::max(1, std::min({threads, k, hw}));
     std::vector<std::string> errs(nt);
     auto work = [&](int tid) {
       try {
