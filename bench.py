@@ -412,8 +412,9 @@ def main():
         log(f"cpu baseline: {cpu['value']:.4f} solves/s ({cpu['sample']})")
 
     mode = "batched+warm" if ws is not None else "batched"
+    tol_label = ("%g" % args.tol).replace("e-0", "e-")  # 1e-06 -> 1e-6
     out = {
-        "metric": f"LP solves/s at rel-KKT<=1e-6 ({mode} refinery family, certified)",
+        "metric": f"LP solves/s at rel-KKT<={tol_label} ({mode} refinery family, certified)",
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
