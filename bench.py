@@ -263,6 +263,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device copy of MIB MiB (units of the HBM traffic counters)")
     ap.add_argument("--calibrate", metavar="JSON", help="run the CPU/GPU crossover sweep and router re-fit only")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="print the standalone SpMV rates on the refinery matrix (one JSON line) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched family shards by "
@@ -273,6 +275,9 @@ def main():
         raise SystemExit(f"no usable gfx950 device: {info}")
     if args.calibrate:
         calibrate(args.cpu_threads, args.calibrate)
+        return
+    if args.spmv_only:
+        print(json.dumps({"spmv": spmv_rates(P.gen_refinery(SEED_LP))}))
         return
     if args.copy_calibration:
         # A plain device-to-device copy of a known size: its FETCH_SIZE /
