@@ -200,3 +200,16 @@ def test_certificate_rejects_wrong_objective():
     out = P.certify_batch(lp, None, np.array([[1.6, 1.2], [2.0, 2.0]]), np.array([[0.4, 0.2], [0.4, 0.2]]),
                           np.array([2.8, 4.0]), threads=2)
     assert out[0, 3] == 1.0 and out[1, 3] == 0.0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N with N > 1 exits with status 2 and
+    says why, before touching a device."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 2
+    assert "one GPU only" in out.stderr and out.stdout == ""
