@@ -22,6 +22,7 @@ Model gen_refinery(uint64_t seed, int T, int S, int K, int P, int Q, double deca
 void gen_family(const Model& base, uint64_t seed, int k, double range, double* thetas, double* costs);
 void family_costs(const Model& base, int k, const double* thetas, double* costs);
 Model gen_calibration(uint64_t seed, int s);
+Model read_mps(const std::string& path, int* sense);
 }  // namespace pdhgamd
 
 struct pdhg_model {
@@ -313,6 +314,13 @@ int pdhg_model_view(const pdhg_model* mdl, pdhg_lp* lp, const double** x_star, c
     lp->obj_offset = m.offset;
     if (x_star) *x_star = m.xstar.empty() ? nullptr : m.xstar.data();
     if (y_star) *y_star = m.ystar.empty() ? nullptr : m.ystar.data();
+  });
+}
+
+pdhg_model* pdhg_read_mps(const char* path, int* sense) {
+  return guard_model([&] {
+    PDHG_REQUIRE(path, "null path");
+    return pdhgamd::read_mps(path, sense);
   });
 }
 

@@ -83,7 +83,7 @@ EXPORTS = [
     "pdhg_backend_diff2", "pdhg_backend_time_spmv",
     "pdhg_safe_dual_bound", "pdhg_certify", "pdhg_certify_batch",
     "pdhg_gen_random_lp", "pdhg_gen_refinery", "pdhg_gen_family", "pdhg_family_costs",
-    "pdhg_gen_calibration", "pdhg_model_view", "pdhg_model_free",
+    "pdhg_gen_calibration", "pdhg_read_mps", "pdhg_model_view", "pdhg_model_free",
     "pdhg_pod_basis", "pdhg_decode_bf16", "pdhg_mlp_create", "pdhg_mlp_destroy", "pdhg_mlp_param_count",
     "pdhg_mlp_get_params", "pdhg_mlp_set_params", "pdhg_mlp_forward", "pdhg_mlp_loss_grad", "pdhg_mlp_train",
 ]
@@ -104,6 +104,8 @@ def lib() -> C.CDLL:
         L.pdhg_gen_random_lp.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_double, C.c_double]
         L.pdhg_gen_refinery.restype = C.c_void_p
         L.pdhg_gen_refinery.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.pdhg_read_mps.restype = C.c_void_p
+        L.pdhg_read_mps.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
         L.pdhg_gen_calibration.restype = C.c_void_p
         L.pdhg_gen_calibration.argtypes = [C.c_uint64, C.c_int]
         L.pdhg_gen_family.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_double, _dp, _dp]
@@ -260,6 +262,16 @@ def gen_random_lp(seed: int, m: int, n: int, nnz_per_col: float = 4.0, row_sigma
 def gen_refinery(seed: int, periods=12, streams=40, crudes=8, products=36, qualities=30, decades=3.0) -> LP:
     """Refinery-style multi-period blending LP (about 5.6e5 nonzeros at the defaults)."""
     return _from_model(lib().pdhg_gen_refinery(seed, periods, streams, crudes, products, qualities, decades))
+
+
+def read_mps(path: str):
+    """A free-format MPS file (plain or gzip) as an LP in minimisation form,
+    and its sense (1, or -1 for OBJSENSE MAX: costs and offset negated)."""
+    sense = C.c_int(1)
+    h = lib().pdhg_read_mps(os.fsencode(path), C.byref(sense))
+    if not h:
+        raise RuntimeError(f"read_mps: {_err()}")
+    return _from_model(h), sense.value
 
 
 def gen_calibration(seed: int, s: int) -> LP:
