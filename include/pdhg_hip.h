@@ -279,12 +279,12 @@ int pdhg_family_costs(const pdhg_model* base, int k, const double* thetas, doubl
  * m = s rows >= 1, n = 2s columns in [0, 10], 4 entries 0.5 + U per column,
  * cost U - 0.3.  Used to fit the CPU/GPU crossover. */
 pdhg_model* pdhg_gen_calibration(uint64_t seed, int s);
-/* Reads a free-format MPS file (plain or gzip) into a model in minimisation
- * form: the LP part of the reference's readMps (src/io/mps.h:15-19) followed
- * by toMinimization (src/core/model.cpp:92-101).  *sense (may be NULL) is -1
- * for an OBJSENSE MAX file, whose costs and offset come back negated, else 1.
- * Quadratic sections, semi-continuous bounds and fixed-format files with
- * blanks in names are rejected (NULL). */
+/* Reads an MPS file (free format, or fixed columns once names contain
+ * blanks; plain or gzip) into a model in minimisation form: the LP part of
+ * the reference's readMps (src/io/mps.h:15-19) followed by toMinimization
+ * (src/core/model.cpp:92-101).  *sense (may be NULL) is -1 for an OBJSENSE
+ * MAX file, whose costs and offset come back negated, else 1.  Quadratic
+ * sections and semi-continuous bounds are rejected (NULL). */
 pdhg_model* pdhg_read_mps(const char* path, int* sense);
 /* Borrowed view of a model; x_star / y_star may be NULL (refinery). */
 int pdhg_model_view(const pdhg_model* mdl, pdhg_lp* lp, const double** x_star, const double** y_star);

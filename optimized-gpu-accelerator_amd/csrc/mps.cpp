@@ -1,10 +1,13 @@
-// Free-format MPS reader feeding the engine (the LP part of the reference's
-// readMps, src/io/mps.h:15-19, src/io/mps.cpp:70-400, followed by
-// toMinimization, src/core/model.cpp:92-101).  Plain or gzip files (zlib reads
-// both).  Sections: NAME, OBJSENSE, ROWS, COLUMNS (integer MARKER lines
-// skipped: the engine solves the LP relaxation), RHS, RANGES, BOUNDS
-// (UP LO FX FR MI PL BV LI UI), ENDATA.  Quadratic sections, semi-continuous
-// bounds and fixed-format files whose names contain blanks are rejected.
+// MPS reader feeding the engine (the LP part of the reference's readMps,
+// src/io/mps.h:15-19, src/io/mps.cpp:40-400, followed by toMinimization,
+// src/core/model.cpp:92-101).  Plain or gzip files (zlib reads both).  Data
+// lines are whitespace-separated; once a ROWS line has more than two fields
+// (names with blanks) the file is read in fixed columns 2-3, 5-12, 15-22,
+// 25-36, 40-47, 50-61, as the reference does.  Sections: NAME, OBJSENSE,
+// ROWS, COLUMNS (integer MARKER lines skipped: the engine solves the LP
+// relaxation), RHS, RANGES, BOUNDS (UP LO FX FR MI PL BV LI UI), ENDATA.
+// Numbers may use Fortran D exponents.  Quadratic sections and
+// semi-continuous bounds are rejected.
 #include <zlib.h>
 
 #include <algorithm>
@@ -35,6 +38,23 @@ std::vector<std::string> tokens(const std::string& s) {
   return t;
 }
 
+std::string trim(const std::string& s) {
+  const size_t b = s.find_first_not_of(" \t");
+  if (b == std::string::npos) return "";
+  return s.substr(b, s.find_last_not_of(" \t") - b + 1);
+}
+
+// The six fixed-format fields of a data line (1-based columns 2-3, 5-12,
+// 15-22, 25-36, 40-47, 50-61), trimmed.
+std::vector<std::string> fixed_fields(const std::string& line) {
+  static const int beg[6] = {1, 4, 14, 24, 39, 49}, len[6] = {2, 8, 8, 12, 8, 12};
+  std::vector<std::string> f(6);
+  const int L = static_cast<int>(line.size());
+  for (int k = 0; k < 6; ++k)
+    if (beg[k] < L) f[k] = trim(line.substr(beg[k], std::min(len[k], L - beg[k])));
+  return f;
+}
+
 std::string upper(std::string s) {
   for (char& ch : s) ch = static_cast<char>(std::toupper(static_cast<unsigned char>(ch)));
   return s;
@@ -59,10 +79,40 @@ struct MpsReader {
     throw Error("MPS " + path + " line " + std::to_string(lineNo) + ": " + what);
   }
   double num(const std::string& s) const {
+    std::string t = s;
+    for (char& ch : t)  // Fortran exponents (1.0D+03) occur in legacy files
+      if (ch == 'D' || ch == 'd') ch = 'E';
     char* end = nullptr;
-    const double v = std::strtod(s.c_str(), &end);
-    if (end == s.c_str() || *end != '\0') fail("bad number '" + s + "'");
+    const double v = std::strtod(t.c_str(), &end);
+    if (end == t.c_str() || *end != '\0') fail("bad number '" + s + "'");
     return v;
+  }
+
+  // A fixed-format data line of section `sec` as the whitespace-token form
+  // the handlers take (set names always present, possibly empty).
+  std::vector<std::string> from_fixed(const std::string& sec, const std::string& line) const {
+    const auto f = fixed_fields(line);
+    std::vector<std::string> t;
+    if (sec == "ROWS") return {f[0], f[1]};
+    if (sec == "COLUMNS") {
+      if (f[2] == "'MARKER'" || f[2] == "MARKER") return {f[1], f[2], f[3]};
+      t = {f[1], f[2], f[3]};
+    } else if (sec == "RHS" || sec == "RANGES") {
+      t = {f[1].empty() ? std::string("SET") : f[1], f[2], f[3]};
+    } else if (sec == "BOUNDS") {
+      const std::string ty = upper(f[0]);
+      const bool noValue = ty == "FR" || ty == "MI" || ty == "PL" || ty == "BV";
+      t = {f[0], f[1].empty() ? std::string("SET") : f[1], f[2]};
+      if (!noValue) t.push_back(f[3]);
+      return t;
+    } else {
+      return {f[0]};
+    }
+    if (!f[4].empty()) {
+      t.push_back(f[4]);
+      t.push_back(f[5]);
+    }
+    return t;
   }
   int row(const std::string& name) const {
     auto it = rowIndex.find(name);
@@ -83,7 +133,7 @@ struct MpsReader {
   }
 
   void rows_line(const std::vector<std::string>& t) {
-    if (t.size() != 2) fail("ROWS line needs a type and a name (blank-containing fixed-format names are not supported)");
+    if (t.size() != 2 || t[1].empty()) fail("ROWS line needs a type and a name");
     const std::string ty = upper(t[0]);
     if (rowIndex.count(t[1])) fail("duplicate row " + t[1]);
     if (ty == "N") {
@@ -217,14 +267,14 @@ Model read_mps(const std::string& path, int* sense) {
   rd.path = path;
   std::string sec, line;
   std::vector<char> buf(1 << 16);
-  bool ended = false;
+  bool ended = false, fixed = false;
   try {
     while (!ended && gzgets(f, buf.data(), static_cast<int>(buf.size()))) {
       line.assign(buf.data());
       ++rd.lineNo;
       while (!line.empty() && (line.back() == '\n' || line.back() == '\r')) line.pop_back();
       if (line.empty() || line[0] == '*') continue;
-      const auto t = tokens(line);
+      auto t = tokens(line);
       if (t.empty()) continue;
       if (!std::isspace(static_cast<unsigned char>(line[0]))) {  // section header
         sec = upper(t[0]);
@@ -237,6 +287,8 @@ Model read_mps(const std::string& path, int* sense) {
         }
         continue;
       }
+      if (sec == "ROWS" && t.size() > 2) fixed = true;  // names with blanks: fixed columns from here on
+      if (fixed && sec != "OBJSENSE" && sec != "OBJSENCE") t = rd.from_fixed(sec, line);
       if (sec == "OBJSENSE" || sec == "OBJSENCE") rd.maximize = upper(t[0]).rfind("MAX", 0) == 0;
       else if (sec == "ROWS") rd.rows_line(t);
       else if (sec == "COLUMNS") rd.columns_line(t);

@@ -265,7 +265,7 @@ def gen_refinery(seed: int, periods=12, streams=40, crudes=8, products=36, quali
 
 
 def read_mps(path: str):
-    """A free-format MPS file (plain or gzip) as an LP in minimisation form,
+    """An MPS file (free or fixed format, plain or gzip) as an LP in minimisation form,
     and its sense (1, or -1 for OBJSENSE MAX: costs and offset negated)."""
     sense = C.c_int(1)
     h = lib().pdhg_read_mps(os.fsencode(path), C.byref(sense))
