@@ -2,7 +2,8 @@
 (src/io/mps.cpp) followed by toMinimization (src/core/model.cpp:92-101).
 
 * The five vendored Netlib models (tests/golden/netlib_*.mps.gz, data files of
-  the reference) read into exactly the arrays of the committed fixtures, which
+  the reference) and the planning model plan_6x4 (a maximisation model) read
+  into exactly the arrays of the committed fixtures, which
   tests/golden/make_netlib_fixtures.py wrote with its own reader.
 * A hand-written file exercises every rule, against arrays worked out by hand:
   OBJSENSE MAX, extra N rows dropped, duplicate entries summed, explicit zeros
@@ -20,7 +21,7 @@ import pytest
 
 import oracle_lib as O
 import pdhg_amd as P
-from conftest import load_netlib
+from conftest import load_netlib, load_plan
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 INF = np.inf
@@ -67,6 +68,18 @@ def test_netlib_files_read_into_the_fixtures(name):
     lp, sense = P.read_mps(os.path.join(GOLDEN, f"netlib_{name}.mps.gz"))
     ref, _ = load_netlib(name)
     assert sense == 1 and (lp.m, lp.n) == (ref.m, ref.n)
+    for f in ("start", "index", "value", "c", "l", "u", "L", "U"):
+        assert np.array_equal(getattr(lp, f), getattr(ref, f)), f
+    assert lp.offset == ref.offset
+
+
+def test_planning_model_reads_into_its_fixture():
+    """plan_6x4 (the reference's data/gen/plan_6x4.mps, gzipped here) is a
+    maximisation model: costs and offset come back negated, as in the fixture
+    tests/golden/make_plan_fixtures.py wrote, with sense -1."""
+    lp, sense = P.read_mps(os.path.join(GOLDEN, "plan_6x4.mps.gz"))
+    ref, _ = load_plan("plan_6x4")
+    assert sense == -1 and (lp.m, lp.n) == (ref.m, ref.n)
     for f in ("start", "index", "value", "c", "l", "u", "L", "U"):
         assert np.array_equal(getattr(lp, f), getattr(ref, f)), f
     assert lp.offset == ref.offset
