@@ -3,8 +3,8 @@
 
 * The five vendored Netlib models (tests/golden/netlib_*.mps.gz, data files of
   the reference) and the planning model plan_6x4 (a maximisation model) read
-  into exactly the arrays of the committed fixtures, which
-  tests/golden/make_netlib_fixtures.py wrote with its own reader.
+  into exactly the arrays of the committed fixtures, which the scripts in
+  tests/golden wrote with their own (Python) reader.
 * A hand-written file exercises every rule, against arrays worked out by hand:
   OBJSENSE MAX, extra N rows dropped, duplicate entries summed, explicit zeros
   dropped, integer markers skipped, RHS on the objective (offset), optional
