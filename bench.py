@@ -1,35 +1,34 @@
 """Benchmark of the restarted-PDHG engine on one MI355X.
 
-Workload (one "step"): 128 held-out cases of the parametric family of the
-refinery-style LP (12 periods x 40 streams x 36 products x 30 qualities,
-555,852 nonzeros, rows/columns scaled by 10^U(-3,3); costs
-c_k = c0 + theta_k d on four crude purchase prices, theta_k ~ U[-0.3, 0.3]^4)
-solved by ONE batched run of width 64 (lanes freed by finished cases take the
-next case) to relative KKT <= 1e-6 from a learned warm start, then every
-answer certified
-(implied bounds + rigorous safe dual bound, acceptance
+Workload (one "step"): the 64-case price family of the refinery-style LP
+(config C3: 12 periods x 40 streams x 36 products x 30 qualities, 555,852
+nonzeros, rows/columns scaled by 10^U(-3,3); costs c_k = c0 + theta_k d on
+four crude purchase prices, theta_k ~ U[-0.3, 0.3]^4, seed 4) solved by ONE
+batched run of width 64 (FP64 iterates, hipGraphs of 64 iterations,
+autotuned row mapping) to relative KKT <= 1e-6 from the learned warm start,
+with the per-case cold-start fallback (a warm case not converged after 256k
+iterations restarts in its lane from the reference's cold start), then every
+answer certified (implied bounds + rigorous safe dual bound, acceptance
 |obj - LB| <= 1e-4 (1 + |obj|)).  A step is: MLP inference theta -> POD
 coefficients (BF16 MFMA), BF16-MFMA decode of the starting points, upload,
 batched solve, download, certification.  The value is certified solves per
-second over the timed steps less their host <-> device transfers (the
-boundary takes host buffers; their time, at copy rates measured in the same
-run, and the rate including them are reported beside it).
+second of wall time over the timed steps (host <-> device transfers
+included).
 
-Before the timer: the warm-start model is fitted once on a separate sample of
-the family (PDHG solutions of 512 other theta at the same tolerance, the
-first 64 from a cold start and the rest warm-started from a model of those;
-POD basis, MLP trained on the GPU); its cost is reported as
-warm_start_fit_seconds.
+Before the timer: the warm-start model is fitted once on 64 other parameters
+of the family (solved at the same tolerance from a cold start; POD basis, MLP
+trained on the GPU); its cost is reported as warm_start_fit_seconds.  Then
+the per-launch roofline of the dominant iteration kernel, the standalone SpMV
+rates, the reference's 16-thread CPU PDHG on a bounded sample of the same
+workload, and plain single-instance GPU PDHG on a sample of the cases.
 
-Also reported: config C4's 256 uneven LPs solved in one packed run of
-persistent kernels (packed_uneven), the same batched solve from a cold start, plain
-single-instance GPU PDHG on a sample of the same instances (solves/s), the
-per-launch roofline of the dominant iteration kernel (algorithmic bytes /
-HIP-event time, HBM peak 8 TB/s), and the reference's own 16-thread CPU PDHG
-(compiled from the reference sources into oracle/_ref) timed on a bounded
-sample of the same batched workload.
+Wall budget: the whole run stays inside --budget seconds (default 540, below
+the 600 s a driver gives it): warmup and timed steps stop early when the
+next step would not fit, and the line reports the steps actually timed.
+After the timed steps, when the budget allows: the same family from a cold
+start (batched_cold) and config C4's 256 uneven LPs in one packed run.
 
-    python bench.py [--steps 1] [--warmup 0] [--gpus 1] [--cold]
+    python bench.py [--steps 20] [--warmup 5] [--gpus 1] [--budget 540]
 """
 import argparse
 import json
@@ -47,6 +46,7 @@ import warmstart as W  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 SEED_LP, SEED_THETA, SEED_TRAIN, SEED_PACKED = 3, 4, 5, 7
+T0 = time.perf_counter()  # process clock of the wall budget
 
 
 def log(*a):
@@ -109,28 +109,6 @@ def family_step(lp, thetas, costs, opts, ws=None, threads=16):
     return res, st, cert, ok
 
 
-def pcie_rates(size=256 << 20):
-    """Host <-> device bandwidth (bytes/s) of one pageable hipMemcpy of `size`
-    bytes each way (after a warm-up copy), the kind of copy the engine makes
-    for case data, starting points and results."""
-    import ctypes
-
-    hip = ctypes.CDLL("libamdhip64.so")
-    host = np.ones(size // 8)
-    dev = ctypes.c_void_p()
-    assert hip.hipMalloc(ctypes.byref(dev), ctypes.c_size_t(size)) == 0
-    hp = host.ctypes.data_as(ctypes.c_void_p)
-    rates = []
-    for kind, (dst, src) in ((1, (dev, hp)), (2, (hp, dev))):  # host to device, device to host
-        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(size), kind) == 0
-        t = time.perf_counter()
-        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(size), kind) == 0
-        assert hip.hipDeviceSynchronize() == 0
-        rates.append(size / (time.perf_counter() - t))
-    hip.hipFree(dev)
-    return rates[0], rates[1]
-
-
 def spmv_rates(lp, reps=200):
     """Standalone CSR SpMV on the scaled matrix: the op-level backend's k_spmv
     (no atomics, rows bucketed by length; the reference's Backend::spmv and
@@ -166,7 +144,7 @@ def cpu_baseline(lp, costs, gpu_iters, seconds, threads):
     family, run for a bounded number of iterations; solves/s extrapolated to the
     iterations its batch would run (every case iterates until the slowest one
     stops, src/pdhg/pdhg.cpp:606-607: the largest per-case count, capped at the
-    iteration limit)."""
+    iteration limit).  The extrapolation factor is stated in the sample."""
     K = costs.shape[0]
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
@@ -176,22 +154,25 @@ def cpu_baseline(lp, costs, gpu_iters, seconds, threads):
     O.ref_set_threads(threads)
     probe = 64
     t = time.perf_counter()
-    r, _, _ = O.ref_solve_batch(lp, costs, tol=1e-30, max_iter=probe)
+    O.ref_solve_batch(lp, costs, tol=1e-30, max_iter=probe)
     per = (time.perf_counter() - t) / probe
     iters = max(probe, int(seconds / max(per, 1e-9)) // 64 * 64)
     t = time.perf_counter()
-    r, _, _ = O.ref_solve_batch(lp, costs, tol=1e-30, max_iter=iters)
+    O.ref_solve_batch(lp, costs, tol=1e-30, max_iter=iters)
     el = time.perf_counter() - t
     per = el / iters
+    factor = gpu_iters / iters
     return {
         "value": K / (per * gpu_iters),
         "unit": "solves/s",
         "cores": threads,
         "kind": "reference",
-        "sample": (f"reference batched CPU PDHG (FP64, {threads} threads) on the same {K}-case family, "
-                   f"{iters} iterations in {el:.1f} s = {per * 1e3:.2f} ms/iteration, extrapolated to the "
-                   f"{gpu_iters} iterations its batch would run; host nproc={os.cpu_count()}"),
+        "sample": (f"the reference's batched CPU PDHG (its own sources, FP64, {threads} threads) on the same "
+                   f"{K}-case family for {iters} iterations in {el:.1f} s = {per * 1e3:.2f} ms per iteration; "
+                   f"solves/s = {K} cases / ({per * 1e3:.2f} ms x {gpu_iters} iterations, the iteration count its "
+                   f"batch would run), an extrapolation by {factor:.0f}x in iterations; host nproc={os.cpu_count()}"),
         "seconds_per_iteration": per,
+        "extrapolation_factor": factor,
     }
 
 
@@ -238,24 +219,73 @@ def calibrate(threads, out_path):
                                                  "fp32": out["fp32"]["crossover_nnz_measured"]}}))
 
 
+def elapsed():
+    return time.perf_counter() - T0
+
+
+def kernel_roofline(lp, costs, prec, row_map, tol):
+    """Per-launch roofline of the iteration kernels at width 64: HIP events on
+    the run's own stream over 200 launches each, after a short solve of the
+    workload's first 64 cases with the tuned mapping."""
+    _, kst, _ = P.solve_batch(lp, costs[:64], P.default_options(tolerance=tol, precision=prec, max_iterations=64 * 8,
+                                                                kernel_timing_reps=200, row_map=row_map))
+    kern = {"k_col": (kst["col_kernel_bytes"], kst["col_kernel_ms"]),
+            "k_row": (kst["row_kernel_bytes"], kst["row_kernel_ms"])}
+    dom = max(kern, key=lambda k: kern[k][1])
+    dbytes, dms = kern[dom]
+    achieved = dbytes / (dms * 1e-3) / 1e9
+    kprec = "fp64" if prec == P.FP64 else "fp32"
+    tiled = kst["col_row_map" if dom == "k_col" else "row_row_map"] == 3  # LDS-staged tiles
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get(f"{dom}{'_tiled' if tiled else ''}_{kprec}")
+    log(f"kernels: k_col {kern['k_col'][1] * 1e3:.1f} us ({kern['k_col'][0] / kern['k_col'][1] / 1e6:.0f} GB/s), "
+        f"k_row {kern['k_row'][1] * 1e3:.1f} us ({kern['k_row'][0] / kern['k_row'][1] / 1e6:.0f} GB/s)")
+    return {"bound": "hbm", "kernel": dom + ("_tiled" if tiled else ""), "achieved": achieved, "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "algorithmic_bytes": dbytes,
+            "launch_us": dms * 1e3,
+            "other_kernel": {k: {"launch_us": v[1] * 1e3, "algorithmic_bytes": v[0]} for k, v in kern.items()
+                             if k != dom}}
+
+
+def plain_gpu(lp, costs, opts, sample):
+    """Plain single-instance GPU PDHG (cold start, one case per run) on the
+    first `sample` cases: certified solves per second."""
+    t = time.perf_counter()
+    ok, iters = 0, []
+    for b in range(sample):
+        r, _, _ = P.solve_batch(lp, costs[b:b + 1], opts)
+        c = P.certify_batch(lp, costs[b:b + 1], r[0].x[None], r[0].y[None], np.array([r[0].objective]))
+        ok += int(r[0].status == P.OPTIMAL and c[0, 3] > 0)
+        iters.append(int(r[0].iterations))
+    el = time.perf_counter() - t
+    return {"solves_per_s": ok / el if sample else None, "cases": sample, "certified": ok, "seconds": el,
+            "iterations": iters}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--budget", type=float, default=540.0,
+                    help="wall seconds for the whole run; warmup and timed steps stop when the next would not fit")
     ap.add_argument("--precision", choices=["fp32", "fp64", "fp32auto"], default="fp64")
     ap.add_argument("--row-map", type=int, default=0, choices=[0, 1, 2, 3],
                     help="row mapping of the iteration kernels: 0 autotuned, 1 buckets, 2 one mapping, 3 LDS tiles")
-    ap.add_argument("--cases", type=int, default=128, help="held-out family cases solved per step")
+    ap.add_argument("--cases", type=int, default=64, help="family cases solved per step (config C3: 64)")
     ap.add_argument("--cold", action="store_true", help="no learned warm start (batched cold start only)")
-    ap.add_argument("--train", type=int, default=512, help="family parameters solved to fit the warm start")
+    ap.add_argument("--train", type=int, default=64, help="family parameters solved to fit the warm start")
     ap.add_argument("--train-tol", type=float, default=1e-6, help="tolerance of the training solves")
     ap.add_argument("--train-steps", type=int, default=3000, help="Adam steps of the MLP")
+    ap.add_argument("--warm-fallback", type=int, default=256_000,
+                    help="iterations after which an unconverged warm-started case restarts from the cold start")
     ap.add_argument("--no-cold-batch", action="store_true", help="skip the cold-start batched comparison run")
     ap.add_argument("--no-packed", action="store_true", help="skip the packed run of 256 uneven LPs")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--plain-sample", type=int, default=2, help="instances solved one by one for the plain baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-iterations", type=int, default=1_000_000,
@@ -265,6 +295,8 @@ def main():
     ap.add_argument("--calibrate", metavar="JSON", help="run the CPU/GPU crossover sweep and router re-fit only")
     ap.add_argument("--spmv-only", action="store_true",
                     help="print the standalone SpMV rates on the refinery matrix (one JSON line) and exit")
+    ap.add_argument("--plain-all", metavar="JSON",
+                    help="solve every case of the family with plain single-instance GPU PDHG, write JSON, exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched family shards by "
@@ -301,8 +333,18 @@ def main():
     K = args.cases
     thetas, costs = P.gen_family(lp, SEED_THETA, K, 0.3)
     opts = P.default_options(tolerance=args.tol, precision=prec, max_iterations=args.max_iterations,
-                             progress_seconds=15.0, row_map=args.row_map)
+                             progress_seconds=15.0, row_map=args.row_map, warm_fallback=args.warm_fallback)
+    plain_opts = P.default_options(tolerance=args.tol, precision=prec, max_iterations=args.max_iterations,
+                                   row_map=args.row_map)
     log(f"workload: refinery m={lp.m} n={lp.n} nnz={lp.nnz}, {K} instances, precision {args.precision}")
+    if args.plain_all:
+        pl = plain_gpu(lp, costs, plain_opts, K)
+        pl.update(workload=f"refinery price family seed {SEED_LP}/{SEED_THETA}, {K} cases, tol {args.tol}, "
+                           f"precision {args.precision}, one case per run (cold start)", gpu=info)
+        with open(args.plain_all, "w") as f:
+            json.dump(pl, f, indent=1)
+        print(json.dumps({"plain_gpu_solves_per_s": pl["solves_per_s"], "cases": K, "file": args.plain_all}))
+        return
 
     ws, fit_s = None, 0.0
     if not args.cold:
@@ -311,151 +353,151 @@ def main():
         fit_s = time.perf_counter() - t
         log(f"warm-start fit: {fit_s:.1f} s ({args.train} parameters at tolerance {args.train_tol})")
 
+    roof = kernel_roofline(lp, costs, prec, args.row_map, args.tol)
+    spmv = spmv_rates(lp)
+    log("spmv (op-level k_spmv, scaled refinery matrix): " + ", ".join(
+        f"{k} A x {v['Ax']['GBs']:.0f} / A^T y {v['ATy']['GBs']:.0f} GB/s" for k, v in spmv.items()))
+    plain = plain_gpu(lp, costs, plain_opts, args.plain_sample)
+    log(f"plain single-instance GPU PDHG: {plain['solves_per_s']} solves/s over {plain['cases']} cases, "
+        f"iterations {plain['iterations']}")
+    plain_full = None
+    pf = os.path.join(ROOT, "profiles", "plain_gpu_c3_fp64.json")
+    if os.path.exists(pf) and prec == P.FP64 and K == 64:
+        d = json.load(open(pf))
+        plain_full = {k: d[k] for k in ("solves_per_s", "cases", "certified", "seconds")}
+
+    # Warmup and timed steps inside the wall budget.
+    step_s = []
+    warm_done = 0
+    reserve = 10.0  # the CPU baseline's and the report's own seconds come after the steps
+    cpu_est = 0.0 if args.no_cpu else 2.2 * args.cpu_seconds
     for w in range(args.warmup):
+        if warm_done >= 1 and elapsed() + (1 + args.steps) * max(step_s) + cpu_est + reserve > args.budget:
+            break  # one warmup at least; more only when every requested timed step still fits
         t = time.perf_counter()
         res, st, cert, ok = family_step(lp, thetas, costs, opts, ws)
-        log(f"warmup {w}: {time.perf_counter() - t:.2f} s, {st['iterations']} iterations, {ok.sum()}/{K} certified")
+        step_s.append(time.perf_counter() - t)
+        warm_done += 1
+        log(f"warmup {w}: {step_s[-1]:.2f} s, {st['iterations']} iterations, {ok.sum()}/{K} certified, "
+            f"{st['warm_fallbacks']} cold fallbacks")
 
-    certified, iters, kkt, gaps, inst_iters = 0, [], [], [], []
+    # The reference's CPU PDHG on the same family, extrapolated to the
+    # iterations of the slowest case of a step (from the warmup when there is one).
+    cpu = None
+
+    def run_cpu(case_iters):
+        c = cpu_baseline(lp, costs, case_iters, args.cpu_seconds, args.cpu_threads)
+        if c:
+            log(f"cpu baseline: {c['value']:.4f} solves/s ({c['sample']})")
+        return c
+
+    if warm_done and not args.no_cpu:
+        cpu = run_cpu(int(min(max(r.iterations for r in res), args.max_iterations)))
+        cpu_est = 0.0
+
+    certified, iters, kkt, gaps, inst_iters, fallbacks = 0, [], [], [], [], 0
     up_bytes = down_bytes = 0
-    t0 = time.perf_counter()
+    timed = []
     for s in range(args.steps):
+        est = max(step_s) if step_s else 0.0
+        if s >= 1 and elapsed() + est + cpu_est + reserve > args.budget:
+            break
         ts = time.perf_counter()
         res, st, cert, ok = family_step(lp, thetas, costs, opts, ws)
+        timed.append(time.perf_counter() - ts)
+        step_s.append(timed[-1])
         up_bytes += st["bytes_uploaded"]
-        # the engine's downloads, and the decoded starting points brought to the host
-        down_bytes += st["bytes_downloaded"] + (K * (lp.n + lp.m) * 8 if ws is not None else 0)
+        down_bytes += st["bytes_downloaded"]
         certified += int(ok.sum())
         iters.append(st["iterations"])
+        fallbacks += st["warm_fallbacks"]
         inst_iters += [r.iterations for r in res]
         kkt.append(max(r.relative_kkt for r in res))
         gaps.append(float(cert[:, 1].max()))
-        log(f"step {s}: {time.perf_counter() - ts:.2f} s, {st['iterations']} iterations, {ok.sum()}/{K} certified, "
-            f"max rel-KKT {kkt[-1]:.2e}, max certified gap {gaps[-1]:.2e}")
-    elapsed = time.perf_counter() - t0
-    # The engine's boundary takes host buffers (as the reference's API does):
-    # each step moves case data, starting points and results over PCIe.  The
-    # value counts the step without that time, at the copy rates measured here.
-    up_bw, down_bw = pcie_rates()
-    pcie_s = up_bytes / up_bw + down_bytes / down_bw
-    value = certified / (elapsed - pcie_s)
-    log(f"host transfers: {up_bytes / 1e6:.1f} MB up, {down_bytes / 1e6:.1f} MB down, {pcie_s * 1e3:.1f} ms at "
-        f"{up_bw / 1e9:.1f} / {down_bw / 1e9:.1f} GB/s")
-    gpu_iters = int(max(iters))  # iterations of the (streamed) run
+        log(f"step {s}: {timed[-1]:.2f} s, {st['iterations']} iterations, {ok.sum()}/{K} certified, "
+            f"{st['warm_fallbacks']} cold fallbacks, max rel-KKT {kkt[-1]:.2e}, max certified gap {gaps[-1]:.2e}")
+    steps = len(timed)
+    elapsed_timed = sum(timed)
+    value = certified / elapsed_timed
+    gpu_iters = int(max(iters))  # iterations of the run
     case_iters = int(min(max(inst_iters), args.max_iterations))  # slowest case, capped
 
-    # The same batched solve from a cold start (one run, for the ratio).
+    if cpu is None and not args.no_cpu:
+        cpu = run_cpu(case_iters)
+
+    # The same batched solve from a cold start, and config C4's uneven batch,
+    # when the budget still holds them.
     cold = None
-    if ws is not None and not args.no_cold_batch:
+    if ws is not None and not args.no_cold_batch and elapsed() + 1.3 * max(step_s) + reserve <= args.budget:
         t = time.perf_counter()
         res_c, st_c, _, ok_c = family_step(lp, thetas, costs, opts, None)
         el = time.perf_counter() - t
         cold = {"solves_per_s": int(ok_c.sum()) / el, "seconds": el, "iterations": st_c["iterations"],
-                "median_instance_iterations": float(np.median([r.iterations for r in res_c]))}
+                "certified": int(ok_c.sum()), "median_instance_iterations": float(np.median([r.iterations for r in res_c]))}
         log(f"batched cold start: {el:.2f} s, {st_c['iterations']} iterations, {ok_c.sum()}/{K} certified")
-
-    # Config C4's uneven batch: 256 unrelated LPs (the config-0 generator, nnz
-    # log-uniform in [2e3, 2e5]) in one packed run of persistent kernels; the
-    # time includes running the certifier on every answer.  These LPs have
-    # x >= 0 without upper bounds (none implied either), so the safe dual
-    # bound is -inf as soon as one reduced cost of a PDHG dual is slightly
-    # negative: the rate counts answers at rel-KKT <= tol, and the certifier's
-    # acceptances are reported beside it.
     packed = None
-    if not args.no_packed:
+    if not args.no_packed and elapsed() + 45.0 + reserve <= args.budget:
+        # Config C4's uneven batch: 256 unrelated LPs (the config-0 generator,
+        # nnz log-uniform in [2e3, 2e5], a non-binding box x <= 2) in one
+        # packed run of persistent kernels, every answer through the certifier.
         lps = P.gen_uneven_batch(SEED_PACKED, 256)
         t = time.perf_counter()
         res_p, st_p = P.solve_packed(lps, P.default_options(tolerance=args.tol, precision=prec,
-                                                            max_iterations=args.max_iterations,
-                                                            progress_seconds=15.0))
+                                                            max_iterations=args.max_iterations))
         conv_p = sum(1 for r in res_p if r.status == P.OPTIMAL)
         cert_p = sum(1 for q, r in zip(lps, res_p)
                      if r.status == P.OPTIMAL and P.certify(q, r.x, r.y, r.objective)["accepted"])
         el = time.perf_counter() - t
         packed = {"lps": len(lps), "nnz": int(sum(q.nnz for q in lps)), "seconds": el, "converged": conv_p,
-                  "converged_per_s": conv_p / el, "certified": cert_p,
+                  "certified": cert_p, "certified_per_s": cert_p / el,
                   "max_instance_iterations": int(max(r.iterations for r in res_p))}
-        log(f"packed uneven batch: {conv_p}/{len(lps)} at rel-KKT <= {args.tol:g} in {el:.2f} s "
-            f"({cert_p} accepted by the certifier)")
-
-    # Kernel roofline: HIP events on the run's own stream, same workload shape.
-    _, kst, _ = P.solve_batch(lp, costs[:64], P.default_options(tolerance=args.tol, precision=prec, max_iterations=64 * 8,
-                                                          kernel_timing_reps=200, row_map=args.row_map))
-    kern = {"k_col": (kst["col_kernel_bytes"], kst["col_kernel_ms"]),
-            "k_row": (kst["row_kernel_bytes"], kst["row_kernel_ms"])}
-    dom = max(kern, key=lambda k: kern[k][1])
-    dbytes, dms = kern[dom]
-    achieved = dbytes / (dms * 1e-3) / 1e9
-    kprec = "fp64" if prec == P.FP64 else "fp32"
-    # Row mapping the tuner chose for that kernel: LDS-staged tiles (3) or rows.
-    tiled = kst["col_row_map" if dom == "k_col" else "row_row_map"] == 3
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get(f"{dom}{'_tiled' if tiled else ''}_{kprec}")
-    log(f"kernels: k_col {kern['k_col'][1] * 1e3:.1f} us ({kern['k_col'][0] / kern['k_col'][1] / 1e6:.0f} GB/s), "
-        f"k_row {kern['k_row'][1] * 1e3:.1f} us ({kern['k_row'][0] / kern['k_row'][1] / 1e6:.0f} GB/s)")
-
-    spmv = spmv_rates(lp)
-    log("spmv (op-level k_spmv, scaled refinery matrix): " + ", ".join(
-        f"{k} A x {v['Ax']['GBs']:.0f} / A^T y {v['ATy']['GBs']:.0f} GB/s" for k, v in spmv.items()))
-
-    # Plain single-instance GPU PDHG (cold start) on a sample of the same instances.
-    tp = time.perf_counter()
-    plain_ok = 0
-    plain_iters = []
-    for b in range(args.plain_sample):
-        r, _, _ = P.solve_batch(lp, costs[b:b + 1], opts)
-        c = P.certify_batch(lp, costs[b:b + 1], r[0].x[None], r[0].y[None], np.array([r[0].objective]))
-        plain_ok += int(r[0].status == P.OPTIMAL and c[0, 3] > 0)
-        plain_iters.append(r[0].iterations)
-    plain = plain_ok / (time.perf_counter() - tp) if args.plain_sample else None
-    log(f"plain single-instance GPU PDHG: {plain} solves/s over {args.plain_sample} instances, iterations {plain_iters}")
-
-    cpu = None if args.no_cpu else cpu_baseline(lp, costs, case_iters, args.cpu_seconds, args.cpu_threads)
-    if cpu:
-        log(f"cpu baseline: {cpu['value']:.4f} solves/s ({cpu['sample']})")
+        log(f"packed uneven batch: {conv_p}/{len(lps)} at rel-KKT <= {args.tol:g}, {cert_p} certified, in {el:.2f} s")
 
     mode = "batched+warm" if ws is not None else "batched"
     tol_label = ("%g" % args.tol).replace("e-0", "e-")  # 1e-06 -> 1e-6
+    plain_rate = plain_full["solves_per_s"] if plain_full else plain["solves_per_s"]
     out = {
-        "metric": f"LP solves/s at rel-KKT<={tol_label} ({mode} refinery family, certified)",
+        "metric": f"LP solves/s at rel-KKT<={tol_label} ({mode}, 64-case refinery price family, certified)",
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "steps": steps,
+        "warmup": warm_done,
+        "ms_per_step": elapsed_timed / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {P.FP64: "f64", P.FP32: "f32", P.FP32_AUTO: "f32+f64"}[prec],
         "data": "synthetic",
-        "config": {"workload": f"refinery-family: m={lp.m} n={lp.n} nnz={lp.nnz}, {K} cost cases streamed "
-                               f"through a run of width 64, tol {args.tol}",
+        "config": {"workload": f"config C3: {K}-case price family of the refinery LP (m={lp.m} n={lp.n} "
+                               f"nnz={lp.nnz}) in one batched run of width 64, tol {args.tol}",
                    "cases": K, "batch_width": 64, "start": "learned warm start" if ws is not None else "cold",
+                   "warm_fallback_iterations": args.warm_fallback if ws is not None else None,
                    "warm_start_training": (f"{args.train} family parameters at rel-KKT <= {args.train_tol:g}"
                                            if ws is not None else None),
                    "iterations": gpu_iters, "median_instance_iterations": float(np.median(inst_iters)),
-                   "certified": certified, "solved": args.steps * K},
-        "roofline": {"bound": "hbm", "kernel": dom + ("_tiled" if tiled else ""), "achieved": achieved,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "algorithmic_bytes": dbytes,
-                     "launch_us": dms * 1e3},
+                   "certified": certified, "solved": steps * K, "cold_fallbacks": fallbacks},
+        "steps_requested": args.steps,
+        "warmup_requested": args.warmup,
+        "budget_s": args.budget,
+        "stopped_by_budget": steps < args.steps or warm_done < args.warmup,
+        "roofline": roof,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "spmv": spmv,
-        "value_with_pcie": certified / elapsed,
-        "pcie": {"seconds": pcie_s, "bytes_up": up_bytes, "bytes_down": down_bytes,
-                 "gb_per_s_up": up_bw / 1e9, "gb_per_s_down": down_bw / 1e9},
-        "plain_gpu_solves_per_s": plain,
-        "vs_plain_gpu": (value / plain) if plain else None,
+        "bytes_per_step": {"uploaded": up_bytes / steps, "downloaded": down_bytes / steps},
+        "plain_gpu_solves_per_s": plain_rate,
+        "plain_gpu_sample": plain,
+        "plain_gpu_all_cases": plain_full,
+        "vs_plain_gpu": (value / plain_rate) if plain_rate else None,
         "batched_cold": cold,
         "packed_uneven": packed,
         "warm_start_fit_seconds": fit_s if ws is not None else None,
         "max_rel_kkt": max(kkt),
         "max_certified_gap": max(gaps),
+        "wall_s": elapsed(),
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+
 
 if __name__ == "__main__":
     main()

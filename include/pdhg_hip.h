@@ -109,6 +109,14 @@ typedef struct pdhg_options {
   int compact;                 /* 1: once half the instances of a batch are done, move the
                                   live ones into a run of half the width (default 1) */
   double progress_seconds;     /* > 0: a progress line on stderr at this interval */
+  long long warm_fallback;     /* cold-start fallback of warm-started cases (x0, y0 or
+                                  primal_weight given): a case not converged after this
+                                  many iterations of its own restarts in its lane from the
+                                  reference's cold start (x0 = clamp(0), y0 = 0, initial
+                                  primal weight), with a full max_iterations of its own;
+                                  its result counts the iterations of both parts.  -1 (the
+                                  default) = max_iterations / 4, rounded up to check_every;
+                                  0 = never */
 } pdhg_options;
 
 /* PdhgStats (src/pdhg/pdhg.h:44-56) plus engine counters. */
@@ -144,6 +152,7 @@ typedef struct pdhg_stats {
   int row_nnz_per_lane, row_row_map, row_xcd;
   double tune_seconds;
   int col_rows_per_wave, row_rows_per_wave; /* B = 64: 1, or 2 rows per wavefront */
+  int warm_fallbacks;          /* warm-started cases restarted from the cold start */
 } pdhg_stats;
 
 /* PdhgResult (src/pdhg/pdhg.h:58-65).  x (n) and row_dual (m) are

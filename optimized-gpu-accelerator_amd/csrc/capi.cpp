@@ -85,6 +85,7 @@ void pdhg_default_options(pdhg_options* o) {
   o->kernel_timing_reps = 0;
   o->compact = 1;
   o->progress_seconds = 0;
+  o->warm_fallback = -1;
 }
 
 int pdhg_gpu_available(void) { return pdhgamd::device_info(0).available ? 1 : 0; }

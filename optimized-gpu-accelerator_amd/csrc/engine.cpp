@@ -77,6 +77,7 @@ struct LaneData {
 
 struct CaseScaled {  // per case: norms of the original vectors and initial omega
   double cNorm = 0, rhsNorm = 0, omega = 1;
+  double omegaCold = 1;  // the reference's initial primal weight (the cold start's)
 };
 
 // ---- autotuner bookkeeping ------------------------------------------------
@@ -163,6 +164,7 @@ void check_options(const pdhg_options& o) {
   PDHG_REQUIRE(o.row_map >= 0 && o.row_map <= 3, "row_map must be in 0..3");
   PDHG_REQUIRE(o.xcd_remap >= -1 && o.xcd_remap <= 1, "xcd_remap must be -1, 0 or 1");
   PDHG_REQUIRE(o.device >= 0 && o.device < kMaxDevices, "device ordinal out of range");
+  PDHG_REQUIRE(o.warm_fallback >= -1, "warm_fallback must be >= -1");
 }
 
 void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pdhg_options& o, pdhg_result* results,
@@ -210,11 +212,28 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     cs[q].cNorm = std::sqrt(cn0);
     cs[q].rhsNorm = std::sqrt(bn0);
     if (cn > 1e-20 && bn > 1e-20) cs[q].omega = std::sqrt(cn) / std::sqrt(bn);
+    cs[q].omegaCold = cs[q].omega;
     if (cases[q].omega0 > 0) cs[q].omega = cases[q].omega0;
   }
+  // Cold-start fallback of warm-started cases: the iteration (of the case's
+  // own count) at which an unconverged one restarts from the cold start.
+  auto is_warm = [&](int q) { return cases[q].x0 || cases[q].y0 || cases[q].omega0 > 0; };
+  long long fallbackAt = 0;
+  if (o.warm_fallback != 0) {
+    const long long w = o.warm_fallback > 0 ? o.warm_fallback : o.max_iterations / 4;
+    fallbackAt = std::max<long long>(o.check_every, (w + o.check_every - 1) / o.check_every * o.check_every);
+    if (fallbackAt >= o.max_iterations) fallbackAt = 0;
+  }
+  bool anyFallback = false;
+  for (int q = 0; q < k && fallbackAt > 0; ++q) anyFallback = anyFallback || is_warm(q);
+  std::vector<char> fellBack(k, 0);
+  std::vector<long long> warmIters(k, 0);  // iterations a case ran warm before its fallback
+  std::vector<int> warmRestarts(k, 0);
+  int fallbacks = 0;
 
-  // Lane data for cases `ids` in a run of width B.
-  auto lane_data = [&](const std::vector<int>& ids, int B) {
+  // Lane data for cases `ids` in a run of width B; `cold` ignores the cases'
+  // warm starts (the fallback).
+  auto lane_data = [&](const std::vector<int>& ids, int B, bool cold = false) {
     LaneData d;
     const size_t nB = static_cast<size_t>(n) * B, mB = static_cast<size_t>(m) * B;
     d.c.assign(shC ? n : nB, 0.0);
@@ -232,7 +251,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       const bool real = b < static_cast<int>(ids.size());
       const int q = ids[real ? b : 0];
       d.pad[b] = real ? 0 : 1;
-      d.omega[b] = cs[q].omega;
+      d.omega[b] = cold ? cs[q].omegaCold : cs[q].omega;
       d.cNorm[b] = cs[q].cNorm;
       d.rhsNorm[b] = cs[q].rhsNorm;
       const double *cc = cost(q), *cl = colL(q), *cu = colU(q), *cL = rowL(q), *cU = rowU(q);
@@ -251,7 +270,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
         }
         // Cold start x0 = clamp(0) (src/pdhg/pdhg.cpp:519), or the warm start
         // mapped into scaled space (inverse of the unscaling at :581), clamped.
-        const double x0 = cases[q].x0 ? cases[q].x0[j] / P.C[j] / P.beta : 0.0;
+        const double x0 = cases[q].x0 && !cold ? cases[q].x0[j] / P.C[j] / P.beta : 0.0;
         d.x[static_cast<size_t>(j) * B + b] = std::min(std::max(x0, lj), uj);
       }
       for (int i = 0; i < m; ++i) {
@@ -264,7 +283,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
           d.rl[i] = li;
           d.ru[i] = ui;
         }
-        if (cases[q].y0) d.y[static_cast<size_t>(i) * B + b] = cases[q].y0[i] / P.R[i] / P.gamma;
+        if (cases[q].y0 && !cold) d.y[static_cast<size_t>(i) * B + b] = cases[q].y0[i] / P.R[i] / P.gamma;
       }
     }
     return d;
@@ -378,13 +397,16 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
   // its own iteration limit) is given the next pending case while the other
   // lanes keep running, so the run's time follows the cases' mean iteration
   // count rather than the slowest case of every group of 64.
-  const bool stream = k > 64;
+  // Lanes run on their own iteration counts (continuous batching) when more
+  // cases than one run holds are queued, and when warm-started cases may
+  // fall back to the cold start.
+  const bool stream = k > 64 || anyFallback;
   for (int first = 0; first < k; first += stream ? k : 64) {
     const auto tChunk = Clock::now();
     std::vector<int> ids;  // lane -> case
     for (int q = first; q < std::min(k, first + 64); ++q) ids.push_back(q);
     std::vector<int> pending;  // cases not yet started (streaming)
-    if (stream)
+    if (k > 64)
       for (int q = k - 1; q >= 64; --q) pending.push_back(q);
     int B = next_pow2(static_cast<int>(ids.size()));
     auto run = make_run(ids, B);
@@ -429,7 +451,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
         const CheckRecord& cr = rec[b];
         if (done[b] && !limitHit[b]) {
           r.status = PDHG_OPTIMAL;
-          r.iterations = doneIter[b];
+          r.iterations = warmIters[q] + doneIter[b];
         } else {
           PDHG_REQUIRE(final || limitHit[b], "internal: extracting a live instance");
           // A lane stopped at its own limit stays so even if the run's time
@@ -437,9 +459,9 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
           const bool timedOut = expired && !limitHit[b];
           r.status = timedOut ? PDHG_TIME_LIMIT : PDHG_ITERATION_LIMIT;
           const long long own = limitHit[b] ? o.max_iterations : it - loadIt[b];
-          r.iterations = timedOut ? own - 1 : own;
+          r.iterations = warmIters[q] + (timedOut ? own - 1 : own);
         }
-        r.restarts = restarts[b];
+        r.restarts = warmRestarts[q] + restarts[b];
         r.objective = cr.pobj;
         r.dual_objective = cr.dobj;
         r.rel_primal = cr.relP;
@@ -451,7 +473,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
           for (int j = 0; j < n; ++j) r.x[j] = xs[static_cast<size_t>(j) * B + b] * P.C[j] * P.beta;
         if (r.row_dual)
           for (int i = 0; i < m; ++i) r.row_dual[i] = ys[static_cast<size_t>(i) * B + b] * P.R[i] * P.gamma;
-        acc.restarts += restarts[b];
+        acc.restarts += warmRestarts[q] + restarts[b];
         extracted[q] = 1;
       }
     };
@@ -466,6 +488,8 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
         if (!done[b]) {
           const long long own = it - loadIt[b];
           nextCheck = std::min({nextCheck, o.check_every - own % o.check_every, o.max_iterations - own});
+          if (fallbackAt > 0 && is_warm(ids[b]) && !fellBack[ids[b]] && own < fallbackAt)
+            nextCheck = std::min(nextCheck, fallbackAt - own);
         }
       const int nIt = static_cast<int>(std::max<long long>(1, nextCheck));
       run->run_block(nIt);
@@ -482,6 +506,32 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
             done[b] = 1;
           }
         run->retire_lanes(capped);
+        // Cold-start fallback: warm-started cases still live at their
+        // fallback point restart in their lanes from the reference's start.
+        std::vector<int> fb, fbIds;
+        for (int b = 0; b < static_cast<int>(ids.size()); ++b) {
+          const int q = ids[b];
+          if (!done[b] && fallbackAt > 0 && is_warm(q) && !fellBack[q] && it - loadIt[b] >= fallbackAt) {
+            fb.push_back(b);
+            fbIds.push_back(q);
+          }
+        }
+        if (!fb.empty()) {
+          const LaneData d = lane_data(fbIds, static_cast<int>(fb.size()), true);
+          run->load_lanes(fb, nullptr, nullptr, nullptr, nullptr, nullptr, d.x.data(), d.y.data(), d.omega.data(),
+                          d.cNorm.data(), d.rhsNorm.data());
+          for (size_t t = 0; t < fb.size(); ++t) {
+            const int b = fb[t], q = fbIds[t];
+            fellBack[q] = 1;
+            warmIters[q] = it - loadIt[b];
+            warmRestarts[q] = restarts[b];
+            ++fallbacks;
+            lastGain[q] = checks;
+            loadIt[b] = it;
+            doneIter[b] = -1;
+            restarts[b] = 0;
+          }
+        }
         // Refill: finished lanes take pending cases, a batch at a time.
         std::vector<int> freeLanes, newly;
         for (int b = 0; b < B; ++b) {
@@ -677,6 +727,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     acc.tune_seconds = tuneSeconds;
     acc.col_rows_per_wave = firstCol.rowsPerWave;
     acc.row_rows_per_wave = firstRow.rowsPerWave;
+    acc.warm_fallbacks = fallbacks;
     *stats = acc;
   }
 }

@@ -176,7 +176,8 @@ class Mlp {
     fill(off_[0], hid, in);
     fill(off_[2], hid, hid);
     fill(off_[4], out, hid);
-    MLP_HIP_OK(hipMemcpy(p_, h.data(), np_ * sizeof(float), hipMemcpyHostToDevice));
+    MLP_HIP_OK(hipMemcpyAsync(p_, h.data(), np_ * sizeof(float), hipMemcpyHostToDevice, s_));
+    MLP_HIP_OK(hipStreamSynchronize(s_));
     reset_adam();
   }
   ~Mlp() {
@@ -185,14 +186,18 @@ class Mlp {
     (void)hipStreamDestroy(s_);
   }
   int params() const { return np_; }
+  // Parameter copies are ordered on the network's own stream, behind any
+  // training step still queued there.
   void get(double* out) {
     std::vector<float> h(np_);
-    MLP_HIP_OK(hipMemcpy(h.data(), p_, np_ * sizeof(float), hipMemcpyDeviceToHost));
+    MLP_HIP_OK(hipMemcpyAsync(h.data(), p_, np_ * sizeof(float), hipMemcpyDeviceToHost, s_));
+    MLP_HIP_OK(hipStreamSynchronize(s_));
     for (int t = 0; t < np_; ++t) out[t] = h[t];
   }
   void set(const double* in) {
     std::vector<float> h(in, in + np_);
-    MLP_HIP_OK(hipMemcpy(p_, h.data(), np_ * sizeof(float), hipMemcpyHostToDevice));
+    MLP_HIP_OK(hipMemcpyAsync(p_, h.data(), np_ * sizeof(float), hipMemcpyHostToDevice, s_));
+    MLP_HIP_OK(hipStreamSynchronize(s_));
   }
   void forward(const double* X, int N, double* Y) {
     MLP_HIP_OK(hipSetDevice(dev_));
@@ -238,8 +243,9 @@ class Mlp {
       for (int it = 0; it < steps; ++it) losses[it] = hl[it];
   }
   void reset_adam() {
-    MLP_HIP_OK(hipMemset(m1_, 0, np_ * sizeof(float)));
-    MLP_HIP_OK(hipMemset(m2_, 0, np_ * sizeof(float)));
+    MLP_HIP_OK(hipMemsetAsync(m1_, 0, np_ * sizeof(float), s_));
+    MLP_HIP_OK(hipMemsetAsync(m2_, 0, np_ * sizeof(float), s_));
+    MLP_HIP_OK(hipStreamSynchronize(s_));
     adamStep_ = 0;
   }
 

@@ -564,6 +564,8 @@ class PackedRun {
   // Iterates, anchors, activities and restart state of slot q (columns
   // col0 .. col0+n, rows row0 .. row0+m) as a one-lane LaneState.
   void export_slot(int q, int col0, int n, int row0, int m, LaneState& ls) {
+    PDHG_REQUIRE(q >= 0 && q < K_ && col0 >= 0 && n >= 0 && col0 + n <= N_ && row0 >= 0 && m >= 0 && row0 + m <= M_,
+                 "packed: exported slot out of range");
     auto get = [&](const T* d, int off, int len, std::vector<double>& out) {
       std::vector<T> t(len);
       HIP_OK(hipMemcpyAsync(t.data(), d + off, len * sizeof(T), hipMemcpyDeviceToHost, s_));
@@ -681,10 +683,13 @@ class PackedRun {
                        dRowBeg_, ps_, tol_, maxIter_, iters, checkEvery_);
     HIP_OK(hipGetLastError());
   }
+  // Host-to-device copies are ordered on the run's own (non-blocking)
+  // stream, behind any launch still queued there, and waited for before the
+  // host vector is released.
   template <typename U>
   U* up(const std::vector<U>& v) {
     U* d = buf_.alloc<U>(v.size());
-    HIP_OK(hipMemcpy(d, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice));
+    putv(d, v);
     return d;
   }
   T* upT(const std::vector<double>& v) {
@@ -693,14 +698,14 @@ class PackedRun {
   }
   void putT(T* d, const std::vector<double>& v) {
     std::vector<T> t(v.begin(), v.end());
-    HIP_OK(hipMemcpy(d, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
+    putv(d, t);
   }
-  void put64(double* d, const std::vector<double>& v) {
-    HIP_OK(hipMemcpy(d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
-  }
+  void put64(double* d, const std::vector<double>& v) { putv(d, v); }
   template <typename U>
   void putv(U* d, const std::vector<U>& v) {
-    HIP_OK(hipMemcpy(d, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice));
+    if (v.empty()) return;
+    HIP_OK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice, s_));
+    HIP_OK(hipStreamSynchronize(s_));
   }
 
   int M_, N_, K_, checkEvery_;

@@ -1505,19 +1505,20 @@ class OpBackend final : public OpBackendBase {
     }
     rfac_ = buf_.alloc<double>(m);
     cfac_ = buf_.alloc<double>(n);
-    HIP_OK(hipMemcpy(rfac_, rfac, sizeof(double) * m, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(cfac_, cfac, sizeof(double) * n, hipMemcpyHostToDevice));
+    put(rfac_, rfac, m);
+    put(cfac_, cfac, n);
     for (int v = 0; v < PDHG_NV; ++v) {
       const size_t len = len_of(v);
       vec_[v] = buf_.alloc<T>(len);
-      HIP_OK(hipMemset(vec_[v], 0, len * sizeof(T)));
+      HIP_OK(hipMemsetAsync(vec_[v], 0, len * sizeof(T), stream_));
     }
     iter_ = buf_.alloc<int>(k);
     mask_ = buf_.alloc<int>(k);
-    HIP_OK(hipMemset(iter_, 0, k * sizeof(int)));
+    HIP_OK(hipMemsetAsync(iter_, 0, k * sizeof(int), stream_));
     tau_ = buf_.alloc<double>(k);
     sigma_ = buf_.alloc<double>(k);
     partial_ = buf_.alloc<double>(static_cast<size_t>(kRed) * k * 3);
+    HIP_OK(hipStreamSynchronize(stream_));
   }
   ~OpBackend() override {
     (void)hipEventDestroy(e0_);
@@ -1525,16 +1526,21 @@ class OpBackend final : public OpBackendBase {
     (void)hipStreamDestroy(stream_);
   }
   size_t len_of(int v) const { return static_cast<size_t>(v >= PDHG_V_Y ? m_ : n_) * k_; }
+  // Every host copy is ordered on the backend's own stream, after the
+  // kernels already queued there (the stream does not synchronise with the
+  // null stream), and waited for before the host buffer is released.
   void set(int v, const double* d) override {
     check_vec(v);
+    PDHG_REQUIRE(d != nullptr, "null vector data");
     std::vector<T> t(d, d + len_of(v));
-    HIP_OK(hipMemcpy(vec_[v], t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
+    put(vec_[v], t.data(), t.size());
   }
   void get(int v, double* d) override {
     check_vec(v);
-    HIP_OK(hipStreamSynchronize(stream_));
+    PDHG_REQUIRE(d != nullptr, "null output");
     std::vector<T> t(len_of(v));
-    HIP_OK(hipMemcpy(t.data(), vec_[v], t.size() * sizeof(T), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpyAsync(t.data(), vec_[v], t.size() * sizeof(T), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
     for (size_t i = 0; i < t.size(); ++i) d[i] = t[i];
   }
   void spmv(int in, int out, bool transpose) override {
@@ -1546,9 +1552,9 @@ class OpBackend final : public OpBackendBase {
     HIP_OK(hipGetLastError());
   }
   void setSteps(const double* tau, const double* sigma) override {
-    HIP_OK(hipStreamSynchronize(stream_));
-    HIP_OK(hipMemcpy(tau_, tau, k_ * sizeof(double), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(sigma_, sigma, k_ * sizeof(double), hipMemcpyHostToDevice));
+    PDHG_REQUIRE(tau && sigma, "null step sizes");
+    put(tau_, tau, k_);
+    put(sigma_, sigma, k_);
   }
   void primal() override {
     const long long len = static_cast<long long>(n_) * k_;
@@ -1575,8 +1581,8 @@ class OpBackend final : public OpBackendBase {
     HIP_OK(hipGetLastError());
   }
   void restart(const int* mask) override {
-    HIP_OK(hipStreamSynchronize(stream_));
-    HIP_OK(hipMemcpy(mask_, mask, k_ * sizeof(int), hipMemcpyHostToDevice));
+    PDHG_REQUIRE(mask != nullptr, "null restart mask");
+    put(mask_, mask, k_);
     auto cp = [&](int rows, int src, int d1, int d2) {
       const long long len = static_cast<long long>(rows) * k_;
       hipLaunchKernelGGL(k_op_restart<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, k_, mask_, vec_[src],
@@ -1640,15 +1646,21 @@ class OpBackend final : public OpBackendBase {
       for (int q = 0; q < g; ++q)
         for (int s = 0; s < 3; ++s) acc[3 * b + s] += host[(static_cast<size_t>(b) * g + q) * 3 + s];
   }
+  template <typename U>
+  void put(U* d, const U* h, size_t count) {
+    if (count == 0) return;
+    HIP_OK(hipMemcpyAsync(d, h, count * sizeof(U), hipMemcpyHostToDevice, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+  }
   int* up(const std::vector<int>& v) {
     int* d = buf_.alloc<int>(v.size());
-    HIP_OK(hipMemcpy(d, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+    put(d, v.data(), v.size());
     return d;
   }
   T* upT(const std::vector<double>& v) {
     std::vector<T> t(v.begin(), v.end());
     T* d = buf_.alloc<T>(v.size());
-    HIP_OK(hipMemcpy(d, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
+    put(d, t.data(), t.size());
     return d;
   }
 
@@ -1700,23 +1712,37 @@ pdhg_backend* pdhg_backend_create(int m, int n, const int* a_ptr, const int* a_i
                                   const double* cfac, int k, int precision, int device) {
   try {
     using namespace pdhgamd;
+    // The inputs are checked before any device work, as model_from checks a
+    // C-ABI LP: the kernels trust row pointers and indices.
+    PDHG_REQUIRE(m >= 0 && n >= 0 && k >= 1, "bad sizes");
+    auto csr = [](int rows, int cols, const int* ptr, const int* idx, const double* val, const char* what) {
+      PDHG_REQUIRE(ptr != nullptr, std::string(what) + ": null row pointers");
+      PDHG_REQUIRE(ptr[0] == 0, std::string(what) + ": row pointers must start at 0");
+      for (int i = 0; i < rows; ++i)
+        PDHG_REQUIRE(ptr[i] <= ptr[i + 1], std::string(what) + ": row pointers not monotone");
+      Csr M;
+      M.rows = rows;
+      M.cols = cols;
+      M.ptr.assign(ptr, ptr + rows + 1);
+      const int nnz = M.ptr.back();
+      PDHG_REQUIRE(nnz == 0 || (idx && val), std::string(what) + ": null index or value array");
+      M.idx.assign(idx, idx + nnz);
+      M.val.assign(val, val + nnz);
+      for (int e = 0; e < nnz; ++e) {
+        PDHG_REQUIRE(M.idx[e] >= 0 && M.idx[e] < cols, std::string(what) + ": column index out of range");
+        PDHG_REQUIRE(std::isfinite(M.val[e]), std::string(what) + ": non-finite value");
+      }
+      return M;
+    };
+    Csr A = csr(m, n, a_ptr, a_idx, a_val, "A");
+    Csr AT = csr(n, m, t_ptr, t_idx, t_val, "A^T");
+    PDHG_REQUIRE(A.nnz() == AT.nnz(), "A and A^T nonzero counts differ");
+    PDHG_REQUIRE((m == 0 || rfac) && (n == 0 || cfac), "null scale factors");
+    for (int i = 0; i < m; ++i) PDHG_REQUIRE(std::isfinite(rfac[i]), "non-finite row factor");
+    for (int j = 0; j < n; ++j) PDHG_REQUIRE(std::isfinite(cfac[j]), "non-finite column factor");
+    PDHG_REQUIRE(precision == PDHG_FP32 || precision == PDHG_FP64, "backend precision must be FP32 or FP64");
     const DeviceInfo& di = device_info(device);
     if (!di.available) throw Error("GPU unavailable: " + di.reason);
-    PDHG_REQUIRE(m >= 0 && n >= 0 && k >= 1, "bad sizes");
-    Csr A, AT;
-    A.rows = m;
-    A.cols = n;
-    A.ptr.assign(a_ptr, a_ptr + m + 1);
-    A.idx.assign(a_idx, a_idx + a_ptr[m]);
-    A.val.assign(a_val, a_val + a_ptr[m]);
-    AT.rows = n;
-    AT.cols = m;
-    AT.ptr.assign(t_ptr, t_ptr + n + 1);
-    AT.idx.assign(t_idx, t_idx + t_ptr[n]);
-    AT.val.assign(t_val, t_val + t_ptr[n]);
-    PDHG_REQUIRE(A.nnz() == AT.nnz(), "A and A^T nonzero counts differ");
-    for (int e = 0; e < A.nnz(); ++e) PDHG_REQUIRE(A.idx[e] >= 0 && A.idx[e] < n, "A column index out of range");
-    for (int e = 0; e < AT.nnz(); ++e) PDHG_REQUIRE(AT.idx[e] >= 0 && AT.idx[e] < m, "A^T column index out of range");
     auto* h = new pdhg_backend;
     if (precision == PDHG_FP64) h->impl = std::make_unique<OpBackend<double>>(m, n, A, AT, rfac, cfac, k, device);
     else h->impl = std::make_unique<OpBackend<float>>(m, n, A, AT, rfac, cfac, k, device);

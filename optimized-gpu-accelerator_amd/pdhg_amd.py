@@ -45,7 +45,8 @@ class Options(C.Structure):
                 ("rescale", C.c_int), ("reflection", C.c_double), ("log_interval", C.c_int),
                 ("precision", C.c_int), ("use_graph", C.c_int), ("lanes_per_row", C.c_int), ("row_map", C.c_int),
                 ("xcd_remap", C.c_int), ("time_limit", C.c_double), ("device", C.c_int),
-                ("kernel_timing_reps", C.c_int), ("compact", C.c_int), ("progress_seconds", C.c_double)]
+                ("kernel_timing_reps", C.c_int), ("compact", C.c_int), ("progress_seconds", C.c_double),
+                ("warm_fallback", C.c_longlong)]
 
 
 class Stats(C.Structure):
@@ -63,7 +64,8 @@ class Stats(C.Structure):
                 ("backend", C.c_char * 32), ("device_name", C.c_char * 64),
                 ("autotuned", C.c_int), ("col_nnz_per_lane", C.c_int), ("col_row_map", C.c_int), ("col_xcd", C.c_int),
                 ("row_nnz_per_lane", C.c_int), ("row_row_map", C.c_int), ("row_xcd", C.c_int),
-                ("tune_seconds", C.c_double), ("col_rows_per_wave", C.c_int), ("row_rows_per_wave", C.c_int)]
+                ("tune_seconds", C.c_double), ("col_rows_per_wave", C.c_int), ("row_rows_per_wave", C.c_int),
+                ("warm_fallbacks", C.c_int)]
 
 
 class ResultStruct(C.Structure):
@@ -600,13 +602,21 @@ def solve_packed(lps: list, opts: Options | None = None):
     return [_result(res[b], xs[b], ys[b]) for b in range(k)], stats_dict(st)
 
 
-def gen_uneven_batch(seed: int, count: int, nnz_lo: float = 2e3, nnz_hi: float = 2e5) -> list:
+def gen_uneven_batch(seed: int, count: int, nnz_lo: float = 2e3, nnz_hi: float = 2e5, upper: float = 2.0) -> list:
     """`count` random LPs of the 1000 x 2000-style generator (m = n / 2, 4 entries per
-    column) with nonzero counts log-uniform in [nnz_lo, nnz_hi]."""
+    column) with nonzero counts log-uniform in [nnz_lo, nnz_hi].  Each gets the
+    column box x <= `upper` (None keeps x >= 0 only).  The generator's optimum has
+    x* <= 1, so a box of 2 leaves (x*, y*) optimal; it gives every column a
+    finite bound, without which the safe dual bound of a PDHG dual is -inf as
+    soon as one reduced cost is slightly negative."""
     rng = np.random.default_rng(seed)
     out = []
     for b in range(count):
         nnz = float(np.exp(rng.uniform(np.log(nnz_lo), np.log(nnz_hi))))
         n = max(8, int(round(nnz / 4)))
-        out.append(gen_random_lp(seed * 100003 + b, max(4, n // 2), n, 4.0, 0.0))
+        lp = gen_random_lp(seed * 100003 + b, max(4, n // 2), n, 4.0, 0.0)
+        if upper is not None:
+            assert lp.xstar is None or float(lp.xstar.max()) < upper
+            lp.u = np.full(lp.n, float(upper))
+        out.append(lp)
     return out

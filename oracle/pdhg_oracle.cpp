@@ -677,4 +677,77 @@ int oracle_safe_dual_bound(int m, int n, const int* start, const int* index, con
   return 0;
 }
 
+// ---- Backend ops one at a time (the Cpu struct above, i.e. CpuBackend,
+// src/pdhg/pdhg.cpp:75-214), for op-level comparison with the device backend.
+struct OracleOps {
+  Csr A, AT;
+  Prepared P;  // only rfac / cfac / m / n are used by Cpu::init
+  Cpu cpu;
+};
+
+void* oracle_ops_create(int m, int n, const int* aPtr, const int* aIdx, const double* aVal, const int* tPtr,
+                        const int* tIdx, const double* tVal, const double* rfac, const double* cfac, int k) {
+  if (m < 0 || n < 0 || k < 1) {
+    gErr = "bad sizes";
+    return nullptr;
+  }
+  auto* h = new OracleOps;
+  h->P.m = m;
+  h->P.n = n;
+  h->P.A.rows = m;
+  h->P.A.cols = n;
+  h->P.A.ptr.assign(aPtr, aPtr + m + 1);
+  h->P.A.idx.assign(aIdx, aIdx + aPtr[m]);
+  h->P.A.val.assign(aVal, aVal + aPtr[m]);
+  h->P.AT.rows = n;
+  h->P.AT.cols = m;
+  h->P.AT.ptr.assign(tPtr, tPtr + n + 1);
+  h->P.AT.idx.assign(tIdx, tIdx + tPtr[n]);
+  h->P.AT.val.assign(tVal, tVal + tPtr[n]);
+  h->P.rfac.assign(rfac, rfac + m);
+  h->P.cfac.assign(cfac, cfac + n);
+  h->cpu.init(h->P, k);
+  return h;
+}
+void oracle_ops_destroy(void* h) { delete static_cast<OracleOps*>(h); }
+int oracle_ops_set(void* h, int vec, const double* d) {
+  auto& c = static_cast<OracleOps*>(h)->cpu;
+  if (vec < 0 || vec >= NV) return -1;
+  std::copy(d, d + c.v[vec].size(), c.v[vec].begin());
+  return 0;
+}
+int oracle_ops_get(void* h, int vec, double* d) {
+  auto& c = static_cast<OracleOps*>(h)->cpu;
+  if (vec < 0 || vec >= NV) return -1;
+  std::copy(c.v[vec].begin(), c.v[vec].end(), d);
+  return 0;
+}
+void oracle_ops_spmv(void* h, int in, int out) { static_cast<OracleOps*>(h)->cpu.spmv(in, out); }
+void oracle_ops_spmvT(void* h, int in, int out) { static_cast<OracleOps*>(h)->cpu.spmvT(in, out); }
+void oracle_ops_set_steps(void* h, const double* tau, const double* sigma) {  // src/pdhg/pdhg.cpp:106-109
+  auto& c = static_cast<OracleOps*>(h)->cpu;
+  c.tau.assign(tau, tau + c.k);
+  c.sigma.assign(sigma, sigma + c.k);
+}
+void oracle_ops_primal(void* h) { static_cast<OracleOps*>(h)->cpu.primal(); }
+void oracle_ops_dual(void* h) { static_cast<OracleOps*>(h)->cpu.dual(); }
+void oracle_ops_halpern(void* h, double rho) { static_cast<OracleOps*>(h)->cpu.halpern(rho); }
+void oracle_ops_restart(void* h, const int* mask) {
+  auto& c = static_cast<OracleOps*>(h)->cpu;
+  c.restart(std::vector<int>(mask, mask + c.k));
+}
+void oracle_ops_kkt(void* h, double* r3, double* c3) {
+  auto& c = static_cast<OracleOps*>(h)->cpu;
+  std::vector<double> r, cc;
+  c.kkt(r, cc);
+  std::copy(r.begin(), r.end(), r3);
+  std::copy(cc.begin(), cc.end(), c3);
+}
+void oracle_ops_diff2(void* h, int a, int b, double* out) {
+  auto& c = static_cast<OracleOps*>(h)->cpu;
+  std::vector<double> o;
+  c.diff2(a, b, o);
+  std::copy(o.begin(), o.end(), out);
+}
+
 }  // extern "C"

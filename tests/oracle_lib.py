@@ -161,3 +161,89 @@ def ref_family_bound(lp, y, cost=None):
                                 _d(lp.L), _d(lp.U), C.c_double(lp.offset), _d(y), C.byref(b))
     assert rc == 0, ref().ref_last_error()
     return b.value
+
+
+class OracleOps:
+    """The oracle's serial CpuBackend restatement (src/pdhg/pdhg.cpp:75-214)
+    driven one op at a time, with the same vector ids and k-interleaved layout
+    as pdhg_amd.Backend."""
+
+    def __init__(self, A_csr, AT_csr, rfac, cfac, k=1):
+        L = oracle()
+        L.oracle_ops_create.restype = C.c_void_p
+        L.oracle_ops_create.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _ip, _ip, _dp, _dp, _dp, C.c_int]
+        for name in ("oracle_ops_destroy", "oracle_ops_primal", "oracle_ops_dual"):
+            getattr(L, name).argtypes = [C.c_void_p]
+        for name in ("oracle_ops_set", "oracle_ops_get"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_int, _dp]
+        for name in ("oracle_ops_spmv", "oracle_ops_spmvT"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.oracle_ops_set_steps.argtypes = [C.c_void_p, _dp, _dp]
+        L.oracle_ops_halpern.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_ops_restart.argtypes = [C.c_void_p, _ip]
+        L.oracle_ops_kkt.argtypes = [C.c_void_p, _dp, _dp]
+        L.oracle_ops_diff2.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        self.L = L
+        self.m, self.n, self.k = A_csr.shape[0], A_csr.shape[1], k
+        self._keep = [np.ascontiguousarray(a, dtype=t) for a, t in (
+            (A_csr.indptr, np.int32), (A_csr.indices, np.int32), (A_csr.data, np.float64),
+            (AT_csr.indptr, np.int32), (AT_csr.indices, np.int32), (AT_csr.data, np.float64),
+            (rfac, np.float64), (cfac, np.float64))]
+        a = self._keep
+        self.h = L.oracle_ops_create(self.m, self.n, _i(a[0]), _i(a[1]), _d(a[2]), _i(a[3]), _i(a[4]), _d(a[5]),
+                                     _d(a[6]), _d(a[7]), k)
+        assert self.h, L.oracle_last_error()
+
+    def close(self):
+        if self.h:
+            self.L.oracle_ops_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _len(self, v):
+        return (self.m if v >= 8 else self.n) * self.k
+
+    def set(self, v, data):
+        a = np.ascontiguousarray(data, dtype=np.float64).ravel()
+        assert a.size == self._len(v)
+        assert self.L.oracle_ops_set(self.h, v, _d(a)) == 0
+
+    def get(self, v):
+        a = np.zeros(self._len(v))
+        assert self.L.oracle_ops_get(self.h, v, _d(a)) == 0
+        return a
+
+    def spmv(self, i, o):
+        self.L.oracle_ops_spmv(self.h, i, o)
+
+    def spmvT(self, i, o):
+        self.L.oracle_ops_spmvT(self.h, i, o)
+
+    def set_steps(self, tau, sigma):
+        t = np.ascontiguousarray(tau, dtype=np.float64)
+        s = np.ascontiguousarray(sigma, dtype=np.float64)
+        self.L.oracle_ops_set_steps(self.h, _d(t), _d(s))
+
+    def primal_step(self):
+        self.L.oracle_ops_primal(self.h)
+
+    def dual_step(self):
+        self.L.oracle_ops_dual(self.h)
+
+    def halpern(self, rho=1.0):
+        self.L.oracle_ops_halpern(self.h, rho)
+
+    def restart(self, mask):
+        mk = np.ascontiguousarray(mask, dtype=np.int32)
+        self.L.oracle_ops_restart(self.h, _i(mk))
+
+    def kkt(self):
+        r3, c3 = np.zeros(3 * self.k), np.zeros(3 * self.k)
+        self.L.oracle_ops_kkt(self.h, _d(r3), _d(c3))
+        return r3, c3
+
+    def diff2(self, a, b):
+        out = np.zeros(self.k)
+        self.L.oracle_ops_diff2(self.h, a, b, _d(out))
+        return out

@@ -295,7 +295,7 @@ def test_lognormal_lp_full_size(gpu, prec, tol):
     generator's closed-form optimum and an independent FP64 host re-evaluation
     of the KKT residuals (its columns have no upper bounds, so the safe dual
     bound without implied bounds is not finite and pins nothing here)."""
-    lp = P.gen_random_lp(21, 50_000, 100_000, 5.12, 1.0)
+    lp = P.gen_random_lp(2, 50_000, 100_000, 5.12, 1.0)  # config C1: seed 2
     obj_star = float(lp.c @ lp.xstar)
     r, st, _ = P.solve(lp, P.default_options(tolerance=tol, precision=prec))
     print(f"prec={prec} tol={tol}: {r.iterations} iterations, {st['solve_seconds']:.2f} s, objective error "

@@ -213,3 +213,55 @@ def test_bench_refuses_more_than_one_gpu():
                          text=True, timeout=300)
     assert out.returncode == 2
     assert "one GPU only" in out.stderr and out.stdout == ""
+
+
+def test_backend_create_rejects_bad_csr_before_device_work():
+    """pdhg_backend_create checks its CSR arrays as model_from checks a C-ABI
+    LP (null arrays, ptr[0] == 0, monotone pointers, index range, finite
+    values and scale factors) before it touches a device, so a malformed call
+    returns NULL with a message instead of reaching the kernels."""
+    import scipy.sparse as sp
+
+    lp = P.gen_random_lp(2, 30, 60)
+    A = lp.scipy_csc().tocsr()
+    AT = A.T.tocsr()
+    rf, cf = np.ones(lp.m), np.ones(lp.n)
+    L = P.lib()
+
+    def create(aptr=None, aidx=None, aval=None, tptr=None, rfac=rf, cfac=cf, precision=P.FP64):
+        arrs = [np.ascontiguousarray(a) for a in (
+            A.indptr if aptr is None else aptr, A.indices if aidx is None else aidx, A.data if aval is None else aval,
+            AT.indptr if tptr is None else tptr, AT.indices, AT.data)]
+        ints = [a.astype(np.int32) for a in (arrs[0], arrs[1], arrs[3], arrs[4])]
+        h = L.pdhg_backend_create(lp.m, lp.n, P._i(ints[0]), P._i(ints[1]), P._d(arrs[2].astype(np.float64)),
+                                  P._i(ints[2]), P._i(ints[3]), P._d(arrs[5].astype(np.float64)),
+                                  None if rfac is None else P._d(rfac), None if cfac is None else P._d(cfac),
+                                  1, precision, 0)
+        assert not h
+        return L.pdhg_last_error().decode()
+
+    p = A.indptr.copy()
+    p[0] = 1
+    assert "start at 0" in create(aptr=p)
+    p = A.indptr.copy()
+    p[5], p[6] = p[6], p[5] - 1  # not monotone
+    assert "not monotone" in create(aptr=p)
+    i = A.indices.copy()
+    i[4] = lp.n
+    assert "column index out of range" in create(aidx=i)
+    v = A.data.copy()
+    v[2] = np.inf
+    assert "non-finite value" in create(aval=v)
+    assert "null scale factors" in create(rfac=None)
+    r = rf.copy()
+    r[3] = np.nan
+    assert "non-finite row factor" in create(rfac=r)
+    assert "backend precision" in create(precision=P.FP32_AUTO)
+    tp = AT.indptr.copy()
+    last = int(np.nonzero(np.diff(tp))[0][-1])  # the last row of A^T with entries
+    tp[last + 1:] -= 1
+    assert "nonzero counts differ" in create(tptr=tp)
+    h = L.pdhg_backend_create(lp.m, lp.n, None, None, None, None, None, None, None, None, 1, P.FP64, 0)
+    assert not h and b"null row pointers" in L.pdhg_last_error()
+    if not P.gpu_available():  # valid input gets as far as the device check
+        assert "GPU unavailable" in create()

@@ -216,3 +216,34 @@ def test_netlib_fixtures_match_highs():
         r = linprog(lp.c, A_ub=A_ub, b_ub=b_ub, A_eq=A[eq], b_eq=lp.L[eq], bounds=bounds, method="highs")
         assert r.status == 0
         assert abs(r.fun + lp.offset - opt) <= 1e-7 * (1 + abs(opt)), (name, r.fun, opt)
+
+
+@pytest.mark.parametrize("which", ["random", "degenerate"])
+def test_host_loop_over_oracle_ops_equals_oracle_solve(which):
+    """The reference host loop restated in tests/pdhg_driver.py, run over the
+    oracle's Backend ops one at a time, gives the oracle's solve bit for bit
+    (iterations, restarts, objective, x, y) on two cost cases: this pins the
+    op-level oracle (compared op by op with the device backend) and the
+    driver (run over the device backend) to the solve that is itself pinned
+    bitwise to the compiled reference."""
+    import pdhg_driver as D
+    from lp_cases import degenerate_costs, degenerate_lp
+
+    if which == "random":
+        lp = P.gen_random_lp(12, 100, 200)
+        rng = np.random.default_rng(4)
+        costs = np.stack([lp.c, lp.c + rng.uniform(0, 0.05, lp.n)])
+    else:
+        lp, _ = degenerate_lp(seed=5, m=60, n=120)
+        costs = degenerate_costs(lp, 2)
+    pr = O.oracle_prepare(lp)
+    A, AT = D.scaled_csr_pair(lp, pr)
+    ops = O.OracleOps(A, AT, pr["beta"] / pr["R"], pr["gamma"] / pr["C"], k=2)
+    res = D.run_pdhg(ops, lp, pr, costs, tol=1e-6, max_iter=100000)
+    ref, xs, ys, _ = O.oracle_solve(lp, costs, tol=1e-6, max_iter=100000)
+    for b in range(2):
+        assert res[b]["status"] == ref[b]["status"] == 1
+        assert res[b]["iterations"] == ref[b]["iterations"]
+        assert res[b]["restarts"] == ref[b]["restarts"]
+        assert res[b]["objective"] == ref[b]["objective"]
+        assert np.array_equal(res[b]["x"], xs[b]) and np.array_equal(res[b]["y"], ys[b])
