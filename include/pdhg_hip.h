@@ -117,6 +117,11 @@ typedef struct pdhg_options {
                                   its result counts the iterations of both parts.  -1 (the
                                   default) = max_iterations / 4, rounded up to check_every;
                                   0 = never */
+  /* Optional stop request, polled on the host at every check (the reference's
+   * Deadline, src/util/timer.h:22-40: a time limit or a cancellation flag);
+   * a nonzero return ends the solve as a time limit does.  NULL = none. */
+  int (*should_stop)(void* user);
+  void* stop_user;
 } pdhg_options;
 
 /* PdhgStats (src/pdhg/pdhg.h:44-56) plus engine counters. */

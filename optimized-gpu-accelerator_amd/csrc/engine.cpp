@@ -585,7 +585,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
                      remaining, real, B, since_s(tSolve));
       }
       if (remaining == 0) break;
-      if (o.time_limit > 0 && since_s(tSetup) >= o.time_limit) {
+      if ((o.time_limit > 0 && since_s(tSetup) >= o.time_limit) || (o.should_stop && o.should_stop(o.stop_user))) {
         expired = true;
         break;
       }

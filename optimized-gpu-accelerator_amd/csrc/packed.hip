@@ -892,7 +892,8 @@ PackedOutcome run_packed(const PackedData& d, const pdhg_options& o, long long m
       std::fprintf(stderr, "  pdhg packed: iteration %lld, %d of %d live\n", r.it, remaining, K);
     }
     if (remaining == 0) break;
-    if (o.time_limit > 0 && std::chrono::duration<double>(Clock::now() - t0).count() >= o.time_limit) {
+    if ((o.time_limit > 0 && std::chrono::duration<double>(Clock::now() - t0).count() >= o.time_limit) ||
+        (o.should_stop && o.should_stop(o.stop_user))) {
       r.expired = true;
       break;
     }
@@ -1032,7 +1033,8 @@ double finish_handed(const PackedData& d, const PackedOutcome& r, const std::vec
         sl.limit = !sl.done;
       }
     }
-    if (o.time_limit > 0 && std::chrono::duration<double>(Clock::now() - t0).count() >= o.time_limit) {
+    if ((o.time_limit > 0 && std::chrono::duration<double>(Clock::now() - t0).count() >= o.time_limit) ||
+        (o.should_stop && o.should_stop(o.stop_user))) {
       expired = true;
       break;
     }

@@ -1478,6 +1478,11 @@ class OpBackendBase {
   virtual double timeSpmv(bool transpose, int reps) = 0;
 };
 
+// Any k from 1 to 64 instances, as the reference's Backend::init takes
+// (src/pdhg/pdhg.cpp:53-55).  The device vectors are interleaved over
+// B = nextpow2(k) lanes (element (i, b) at i*B + b, the layout of the SpMV
+// kernels); set / get convert from and to the caller's k-interleaved arrays,
+// and the B - k padding lanes (zero data, unit steps) are never reported.
 template <typename T>
 class OpBackend final : public OpBackendBase {
  public:
@@ -1487,8 +1492,9 @@ class OpBackend final : public OpBackendBase {
     HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     HIP_OK(hipEventCreate(&e0_));
     HIP_OK(hipEventCreate(&e1_));
+    PDHG_REQUIRE(k >= 1 && k <= 64, "backend batch width must be in 1..64");
     logB_ = ilog2(k);
-    PDHG_REQUIRE((1 << logB_) == k && k <= 64, "backend batch width must be a power of two <= 64");
+    B_ = 1 << logB_;
     kern_ = kern_table<T>(logB_);
     aPtr_ = up(A.ptr);
     aIdx_ = up(A.idx);
@@ -1508,15 +1514,19 @@ class OpBackend final : public OpBackendBase {
     put(rfac_, rfac, m);
     put(cfac_, cfac, n);
     for (int v = 0; v < PDHG_NV; ++v) {
-      const size_t len = len_of(v);
+      const size_t len = dev_len(v);
       vec_[v] = buf_.alloc<T>(len);
       HIP_OK(hipMemsetAsync(vec_[v], 0, len * sizeof(T), stream_));
     }
-    iter_ = buf_.alloc<int>(k);
-    mask_ = buf_.alloc<int>(k);
-    HIP_OK(hipMemsetAsync(iter_, 0, k * sizeof(int), stream_));
-    tau_ = buf_.alloc<double>(k);
-    sigma_ = buf_.alloc<double>(k);
+    iter_ = buf_.alloc<int>(B_);
+    mask_ = buf_.alloc<int>(B_);
+    HIP_OK(hipMemsetAsync(iter_, 0, B_ * sizeof(int), stream_));
+    HIP_OK(hipMemsetAsync(mask_, 0, B_ * sizeof(int), stream_));
+    tau_ = buf_.alloc<double>(B_);
+    sigma_ = buf_.alloc<double>(B_);
+    const std::vector<double> one(B_, 1.0);
+    put(tau_, one.data(), B_);
+    put(sigma_, one.data(), B_);
     partial_ = buf_.alloc<double>(static_cast<size_t>(kRed) * k * 3);
     HIP_OK(hipStreamSynchronize(stream_));
   }
@@ -1525,23 +1535,30 @@ class OpBackend final : public OpBackendBase {
     (void)hipEventDestroy(e1_);
     (void)hipStreamDestroy(stream_);
   }
-  size_t len_of(int v) const { return static_cast<size_t>(v >= PDHG_V_Y ? m_ : n_) * k_; }
+  int rows_of(int v) const { return v >= PDHG_V_Y ? m_ : n_; }
+  size_t len_of(int v) const { return static_cast<size_t>(rows_of(v)) * k_; }   // caller's layout
+  size_t dev_len(int v) const { return static_cast<size_t>(rows_of(v)) * B_; }  // device layout
   // Every host copy is ordered on the backend's own stream, after the
   // kernels already queued there (the stream does not synchronise with the
   // null stream), and waited for before the host buffer is released.
   void set(int v, const double* d) override {
     check_vec(v);
     PDHG_REQUIRE(d != nullptr, "null vector data");
-    std::vector<T> t(d, d + len_of(v));
+    std::vector<T> t(dev_len(v), static_cast<T>(0));
+    const int rows = rows_of(v);
+    for (int i = 0; i < rows; ++i)
+      for (int b = 0; b < k_; ++b) t[static_cast<size_t>(i) * B_ + b] = static_cast<T>(d[static_cast<size_t>(i) * k_ + b]);
     put(vec_[v], t.data(), t.size());
   }
   void get(int v, double* d) override {
     check_vec(v);
     PDHG_REQUIRE(d != nullptr, "null output");
-    std::vector<T> t(len_of(v));
+    std::vector<T> t(dev_len(v));
     HIP_OK(hipMemcpyAsync(t.data(), vec_[v], t.size() * sizeof(T), hipMemcpyDeviceToHost, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
-    for (size_t i = 0; i < t.size(); ++i) d[i] = t[i];
+    const int rows = rows_of(v);
+    for (int i = 0; i < rows; ++i)
+      for (int b = 0; b < k_; ++b) d[static_cast<size_t>(i) * k_ + b] = t[static_cast<size_t>(i) * B_ + b];
   }
   void spmv(int in, int out, bool transpose) override {
     check_vec(in);
@@ -1553,45 +1570,45 @@ class OpBackend final : public OpBackendBase {
   }
   void setSteps(const double* tau, const double* sigma) override {
     PDHG_REQUIRE(tau && sigma, "null step sizes");
-    put(tau_, tau, k_);
+    put(tau_, tau, k_);  // padding lanes keep unit steps
     put(sigma_, sigma, k_);
   }
   void primal() override {
-    const long long len = static_cast<long long>(n_) * k_;
-    hipLaunchKernelGGL(k_op_primal<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, k_, vec_[PDHG_V_X],
+    const long long len = static_cast<long long>(n_) * B_;
+    hipLaunchKernelGGL(k_op_primal<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, B_, vec_[PDHG_V_X],
                        vec_[PDHG_V_ATY], vec_[PDHG_V_C], vec_[PDHG_V_LO], vec_[PDHG_V_UP], tau_, vec_[PDHG_V_XP]);
     HIP_OK(hipGetLastError());
   }
   void dual() override {
-    const long long len = static_cast<long long>(m_) * k_;
-    hipLaunchKernelGGL(k_op_dual<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, k_, vec_[PDHG_V_Y],
+    const long long len = static_cast<long long>(m_) * B_;
+    hipLaunchKernelGGL(k_op_dual<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, B_, vec_[PDHG_V_Y],
                        vec_[PDHG_V_AXP], vec_[PDHG_V_AX], vec_[PDHG_V_RL], vec_[PDHG_V_RU], sigma_, vec_[PDHG_V_YP]);
     HIP_OK(hipGetLastError());
   }
   void halpern(double rho) override {
     auto one = [&](int rows, int z, int zp, int z0) {
-      const long long len = static_cast<long long>(rows) * k_;
-      hipLaunchKernelGGL(k_op_halpern<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, k_, iter_, rho, vec_[z],
+      const long long len = static_cast<long long>(rows) * B_;
+      hipLaunchKernelGGL(k_op_halpern<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, B_, iter_, rho, vec_[z],
                          vec_[zp], vec_[z0]);
     };
     one(n_, PDHG_V_X, PDHG_V_XP, PDHG_V_X0);
     one(m_, PDHG_V_Y, PDHG_V_YP, PDHG_V_Y0);
     one(m_, PDHG_V_AX, PDHG_V_AXP, PDHG_V_AX0);
-    hipLaunchKernelGGL(k_op_advance, dim3(1), dim3(64), 0, stream_, k_, iter_, mask_, 0);
+    hipLaunchKernelGGL(k_op_advance, dim3(1), dim3(64), 0, stream_, B_, iter_, mask_, 0);
     HIP_OK(hipGetLastError());
   }
   void restart(const int* mask) override {
     PDHG_REQUIRE(mask != nullptr, "null restart mask");
     put(mask_, mask, k_);
     auto cp = [&](int rows, int src, int d1, int d2) {
-      const long long len = static_cast<long long>(rows) * k_;
-      hipLaunchKernelGGL(k_op_restart<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, k_, mask_, vec_[src],
+      const long long len = static_cast<long long>(rows) * B_;
+      hipLaunchKernelGGL(k_op_restart<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, B_, mask_, vec_[src],
                          vec_[d1], vec_[d2]);
     };
     cp(n_, PDHG_V_XP, PDHG_V_X, PDHG_V_X0);
     cp(m_, PDHG_V_YP, PDHG_V_Y, PDHG_V_Y0);
     cp(m_, PDHG_V_AXP, PDHG_V_AX, PDHG_V_AX0);
-    hipLaunchKernelGGL(k_op_advance, dim3(1), dim3(64), 0, stream_, k_, iter_, mask_, 1);
+    hipLaunchKernelGGL(k_op_advance, dim3(1), dim3(64), 0, stream_, B_, iter_, mask_, 1);
     HIP_OK(hipGetLastError());
   }
   void kkt(double* r3, double* c3) override {
@@ -1600,12 +1617,12 @@ class OpBackend final : public OpBackendBase {
     std::vector<double> host;
     if (m_ > 0) {
       const int g = red_blocks(m_);
-      hipLaunchKernelGGL(k_op_kkt_rows<T>, dim3(g, k_), dim3(kWG), 0, stream_, m_, k_, vec_[PDHG_V_AXP],
+      hipLaunchKernelGGL(k_op_kkt_rows<T>, dim3(g, k_), dim3(kWG), 0, stream_, m_, B_, vec_[PDHG_V_AXP],
                          vec_[PDHG_V_RL], vec_[PDHG_V_RU], vec_[PDHG_V_YP], rfac_, partial_);
       fetch_sum3(g, r3);
     }
     const int g = red_blocks(n_);
-    hipLaunchKernelGGL(k_op_kkt_cols<T>, dim3(g, k_), dim3(kWG), 0, stream_, n_, k_, vec_[PDHG_V_XP],
+    hipLaunchKernelGGL(k_op_kkt_cols<T>, dim3(g, k_), dim3(kWG), 0, stream_, n_, B_, vec_[PDHG_V_XP],
                        vec_[PDHG_V_C], vec_[PDHG_V_ATYP], vec_[PDHG_V_LO], vec_[PDHG_V_UP], cfac_, partial_);
     fetch_sum3(g, c3);
   }
@@ -1616,7 +1633,7 @@ class OpBackend final : public OpBackendBase {
     std::fill(out, out + k_, 0.0);
     if (len == 0) return;
     const int g = red_blocks(len);
-    hipLaunchKernelGGL(k_op_diff2<T>, dim3(g, k_), dim3(kWG), 0, stream_, len, k_, vec_[a], vec_[b], partial_);
+    hipLaunchKernelGGL(k_op_diff2<T>, dim3(g, k_), dim3(kWG), 0, stream_, len, B_, vec_[a], vec_[b], partial_);
     std::vector<double> r3(3 * k_, 0.0);
     fetch_sum3(g, r3.data());
     for (int q = 0; q < k_; ++q) out[q] = r3[3 * q];
@@ -1664,7 +1681,7 @@ class OpBackend final : public OpBackendBase {
     return d;
   }
 
-  int m_, n_, k_, logB_ = 0;
+  int m_, n_, k_, logB_ = 0, B_ = 1;
   hipStream_t stream_ = nullptr;
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
   Buffers buf_;

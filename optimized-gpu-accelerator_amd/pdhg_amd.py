@@ -46,7 +46,7 @@ class Options(C.Structure):
                 ("precision", C.c_int), ("use_graph", C.c_int), ("lanes_per_row", C.c_int), ("row_map", C.c_int),
                 ("xcd_remap", C.c_int), ("time_limit", C.c_double), ("device", C.c_int),
                 ("kernel_timing_reps", C.c_int), ("compact", C.c_int), ("progress_seconds", C.c_double),
-                ("warm_fallback", C.c_longlong)]
+                ("warm_fallback", C.c_longlong), ("should_stop", C.c_void_p), ("stop_user", C.c_void_p)]
 
 
 class Stats(C.Structure):

@@ -100,3 +100,96 @@ def refit(rows: list[dict], key: str = "gpu") -> dict:
             "crossover_nnz_fit": fit_cross,
             "crossover_nnz_measured": measured_crossover(nnz, tc, tg),
             "weights": w}
+
+
+# ---------------------------------------------------------------------------
+# Full router re-fit from measured solve times (bench/fit_router.py:30-126)
+# ---------------------------------------------------------------------------
+FEATURES = ["1", "log(m+n)", "log(nnz)", "log(nnz/row)", "log(max row nnz)", "log(coef range)", "eq fraction",
+            "free fraction"]
+
+
+def phi(f: dict) -> list[float]:
+    """Feature vector of the router (RouterFeatures::phi, src/router/router.cpp:24-28;
+    bench/fit_router.py:30-33): logs floored at log 1 = 0."""
+    lg = lambda v: math.log(max(float(v or 0), 1.0))  # noqa: E731
+    return [1.0, lg(f["rows"] + f["cols"]), lg(f["nnz"]), lg(f["avg_row_nnz"]), lg(f["max_row_nnz"]),
+            lg(f["coefficient_range"]), float(f["equality_row_fraction"]), float(f["free_column_fraction"])]
+
+
+def ridge(X, y, lam: float = 1e-2) -> np.ndarray:
+    """argmin_w ||X w - y||^2 + lam ||w||^2 (bench/fit_router.py:68-72)."""
+    X = np.asarray(X, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    return np.linalg.solve(X.T @ X + lam * np.eye(X.shape[1]), X.T @ y)
+
+
+def predicted_seconds(w, f: dict) -> float:
+    return math.exp(float(np.dot(w, phi(f))))
+
+
+def decide(weights: dict, f: dict, gpu: bool = True, tolerance: float = 1e-6) -> str:
+    """The reference router's LP choice (Router::decide, src/router/router.cpp:157-203):
+    minimum predicted time with a crossover polish of 0.15 x the dual simplex
+    prediction on the first-order engines, x4 for those below tolerance 1e-7,
+    x1.1 on the IPM (its crossover)."""
+    t = {e: predicted_seconds(weights[e], f) for e in ENGINES}
+    if not gpu:
+        t["pdhg-gpu"] = math.inf
+    polish = 0.15 * t["dual"]
+    tight = 4.0 if tolerance < 1e-7 else 1.0
+    opts = [("dual", t["dual"]), ("ipm", t["ipm"] * 1.1), ("pdhg-cpu", t["pdhg-cpu"] * tight + polish),
+            ("pdhg-gpu", t["pdhg-gpu"] * tight + polish)]
+    return min(opts, key=lambda o: o[1])[0]
+
+
+def censored_time(seconds, proven: bool, limit: float) -> float:
+    """A run that did not end with a proof counts 4x its time limit (bench/fit_router.py:55-57)."""
+    return float(seconds) if proven and seconds is not None else 4.0 * limit
+
+
+def fit_router(records: list[dict], lam: float = 1e-2, folds: int = 5, seed: int = 1) -> dict:
+    """Per-engine ridge fit of log(t + 1e-3) on phi (bench/fit_router.py:75-126).
+
+    records: {"instance", "family", "features" (the router's feature record),
+    "times": {engine: censored seconds}}.  Validation: `folds`-fold cross
+    validation with each family's instances shuffled and dealt round-robin;
+    reports the geometric-mean and worst regret time(chosen) / time(fastest)
+    of the held-out predictions and of each fixed engine, and how often the
+    fastest engine is picked.  The final model is fitted to all records."""
+    import random
+
+    names = sorted(r["instance"] for r in records)
+    data = {r["instance"]: r for r in records}
+    engines = [e for e in ENGINES if all(e in data[n]["times"] for n in names)]
+    rng = random.Random(seed)
+    fold = {}
+    for fam in sorted({r["family"] for r in records}):
+        members = [n for n in names if data[n]["family"] == fam]
+        rng.shuffle(members)
+        for i, n in enumerate(members):
+            fold[n] = i % folds
+    X = {n: phi(data[n]["features"]) for n in names}
+    y = {n: {e: math.log(data[n]["times"][e] + 1e-3) for e in engines} for n in names}
+    chosen = {}
+    for f in range(folds):
+        train = [n for n in names if fold[n] != f]
+        test = [n for n in names if fold[n] == f]
+        if not train or not test:
+            continue
+        W = {e: ridge([X[n] for n in train], [y[n][e] for n in train], lam) for e in engines}
+        for n in test:
+            pred = {e: float(np.dot(W[e], X[n])) for e in engines}
+            chosen[n] = min(pred, key=pred.get)
+
+    def regret(sel):
+        r = [(data[n]["times"][sel[n]] + 1e-3) / (min(data[n]["times"][e] for e in engines) + 1e-3) for n in sel]
+        return {"geo_mean": math.exp(sum(math.log(v) for v in r) / len(r)), "worst": max(r),
+                "picks_fastest": sum(1 for v in r if v <= 1.0001) / len(r)}
+
+    W = {e: ridge([X[n] for n in names], [y[n][e] for n in names], lam).tolist() for e in engines}
+    for e in ENGINES:
+        W.setdefault(e, [50.0] + [0.0] * 7)  # an engine without data is never chosen
+    return {"weights": W, "features": FEATURES, "instances": len(names), "engines": engines,
+            "validation": {"router_held_out": regret(chosen) if chosen else None,
+                           **{f"always_{e}": regret({n: e for n in names}) for e in engines}}}

@@ -14,8 +14,9 @@ dual bound, |obj - LB| <= 1e-4 (1 + |obj|)).
 
 C4's uneven batch: 256 random LPs of the config-0 generator (seed 7, nnz
 log-uniform in [2e3, 2e5], a non-binding box x <= 2) in one packed run:
-each converged answer against its generator's closed-form optimum, the host
-KKT re-evaluation and the certifier."""
+each converged answer against its generator's closed-form optimum (relative
+to the size of the objective's terms), the host KKT re-evaluation and the
+certifier."""
 import numpy as np
 import pytest
 
@@ -116,7 +117,11 @@ def test_c4_uneven_batch_full_size(gpu):
             continue
         conv += 1
         obj_star = float(lp.c @ lp.xstar) + lp.offset
-        assert abs(r.objective - obj_star) <= 1e-5 * (1 + abs(obj_star)), (lp.m, lp.n)
+        # These objectives are sums of terms of both signs that nearly cancel
+        # (|c'x*| ~ 0.3 against sum |c_j x*_j| ~ 10^2): the error is measured
+        # against the size of the terms.
+        scale = 1 + float(np.abs(lp.c * lp.xstar).sum())
+        assert abs(r.objective - obj_star) <= 1e-6 * scale, (lp.m, lp.n, r.objective, obj_star)
         relP, relD, gap, _ = host_kkt(lp, r.x, r.y)
         assert max(relP, relD, gap) <= 1.5e-6, (lp.m, lp.n, relP, relD, gap)
         c = P.certify(lp, r.x, r.y, r.objective)

@@ -51,3 +51,73 @@ def test_crossover_edge_cases():
     x = R.measured_crossover(nnz, [1, 1, 1], [4, 2, 0.5])
     r1, r2 = math.log(2), math.log(0.5)
     assert x == pytest.approx(math.exp(math.log(1e4) + r1 / (r1 - r2) * (math.log(1e5) - math.log(1e4))))
+
+
+def test_ridge_is_augmented_least_squares():
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((30, 8))
+    y = rng.standard_normal(30)
+    lam = 0.37
+    w = R.ridge(X, y, lam)
+    Xa = np.vstack([X, math.sqrt(lam) * np.eye(8)])
+    ya = np.concatenate([y, np.zeros(8)])
+    assert np.allclose(w, np.linalg.lstsq(Xa, ya, rcond=None)[0], rtol=1e-10, atol=1e-12)
+
+
+def test_fit_router_recovers_planted_weights_and_zero_regret():
+    """Times generated exactly by planted per-engine weights: the fit recovers
+    them (ridge term tiny) and the held-out choices are always the fastest."""
+    rng = np.random.default_rng(5)
+    planted = {e: rng.uniform(-1, 1, 8) * [3, 0.5, 0.5, 0.3, 0.3, 0.2, 1, 1] for e in R.ENGINES}
+    records = []
+    for q in range(60):
+        f = {"rows": int(rng.integers(10, 10 ** 6)), "cols": int(rng.integers(10, 10 ** 6)),
+             "nnz": int(rng.integers(100, 10 ** 7)), "avg_row_nnz": float(rng.uniform(1, 50)),
+             "max_row_nnz": float(rng.uniform(1, 500)), "coefficient_range": float(10 ** rng.uniform(0, 8)),
+             "equality_row_fraction": float(rng.uniform()), "free_column_fraction": float(rng.uniform())}
+        t = {e: math.exp(float(np.dot(planted[e], R.phi(f)))) - 1e-3 for e in R.ENGINES}
+        if min(t.values()) <= 0:
+            continue
+        records.append({"instance": f"i{q}", "family": "gen" if q % 2 else "netlib", "features": f, "times": t})
+    out = R.fit_router(records, lam=1e-9)
+    for e in R.ENGINES:
+        assert np.allclose(out["weights"][e], planted[e], atol=1e-5)
+    v = out["validation"]["router_held_out"]
+    assert v["geo_mean"] == pytest.approx(1.0, abs=1e-6) and v["picks_fastest"] == 1.0
+
+
+def test_censoring_rule():
+    assert R.censored_time(2.5, True, 30.0) == 2.5
+    assert R.censored_time(2.5, False, 30.0) == 120.0
+    assert R.censored_time(None, True, 30.0) == 120.0
+
+
+def test_phi_and_decision_match_the_reference_router(tmp_path):
+    """The reference's own Router (src/router/router.cpp:24-28,157-203), loaded
+    with random weights through --router-model in the reference CLI built over
+    this engine: its predicted seconds equal exp(w . phi) of this module's phi
+    on the features it reports, and its choice equals decide() (no GPU here,
+    so pdhg-gpu is not available to either)."""
+    import json
+
+    import integration_lib as I
+
+    if not I.available():
+        pytest.skip("integration build absent (needs the reference tree)")
+    import pdhg_amd as P
+
+    if P.gpu_available():
+        pytest.skip("decisions without a GPU are compared here")
+    rng = np.random.default_rng(11)
+    for name in ("afiro", "sc105", "kb2", "share1b"):
+        w = {e: (rng.uniform(-1, 1, 8) * [4, 0.5, 0.5, 0.3, 0.3, 0.2, 1, 1]).tolist() for e in R.ENGINES}
+        model = tmp_path / "router_model.json"
+        model.write_text(json.dumps({"weights": w}))
+        out = tmp_path / f"{name}.json"
+        rc, stdout, stderr = I.cli(I.netlib_path(name), "--router-model", str(model), "--json", str(out), "--log", "0")
+        assert rc == 0, stderr
+        tel = json.loads(out.read_text())["telemetry"]["router"]
+        assert tel["calibrated_model"] is True
+        for e in ("dual", "ipm", "pdhg-cpu"):
+            assert tel["predicted_seconds"][e] == pytest.approx(R.predicted_seconds(w[e], tel["features"]), rel=1e-12)
+        assert tel["engine"] == R.decide(w, tel["features"], gpu=False)
