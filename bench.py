@@ -219,6 +219,50 @@ def calibrate(threads, out_path):
                                                  "fp32": out["fp32"]["crossover_nnz_measured"]}}))
 
 
+def large_lp_rates(periods, reps=50):
+    """Kernels on a refinery LP too large for the 256 MB Infinity Cache
+    (`periods` periods of the staircase: 1200 gives 55.6M nonzeros, 445 MB of
+    FP32 values and indices per matrix): the op-level CSR SpMV at width 1
+    (A x and A^T y, FP32 and FP64 values), and the fused iteration kernels of
+    one instance (FP32 and FP64 iterates), HIP events over `reps` launches."""
+    lp = P.gen_refinery(SEED_LP, periods=periods)
+    log(f"large LP: {periods} periods, m={lp.m} n={lp.n} nnz={lp.nnz}")
+    import scipy.sparse as sp
+
+    pr = P.prepare(lp)
+    A = sp.csr_matrix((pr["val"], pr["idx"], pr["ptr"]), shape=(lp.m, lp.n))
+    AT = A.T.tocsr()
+    AT.sort_indices()
+    rfac, cfac = pr["beta"] / pr["R"], pr["gamma"] / pr["C"]
+    rng = np.random.default_rng(0)
+    out = {"workload": f"refinery LP, {periods} periods: m={lp.m} n={lp.n} nnz={lp.nnz}", "spmv": {}, "iteration": {}}
+    for prec, name, vs in ((P.FP32, "fp32", 4), (P.FP64, "fp64", 8)):
+        be = P.Backend(A, AT, rfac, cfac, k=1, precision=prec)
+        be.set(P.V_X, rng.standard_normal(lp.n))
+        be.set(P.V_Y, rng.standard_normal(lp.m))
+        r = {}
+        for transpose, key, rows, cols in ((False, "Ax", lp.m, lp.n), (True, "ATy", lp.n, lp.m)):
+            ms = be.time_spmv(transpose, reps)
+            nbytes = lp.nnz * (vs + 4) + (rows + 1) * 4 + (rows + cols) * vs
+            r[key] = {"us": ms * 1e3, "GBs": nbytes / (ms * 1e-3) / 1e9, "bytes": nbytes,
+                      "frac": nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+        be.close()
+        out["spmv"][name] = r
+        log(f"large-LP spmv {name}: A x {r['Ax']['us']:.1f} us {r['Ax']['GBs']:.0f} GB/s, "
+            f"A^T y {r['ATy']['us']:.1f} us {r['ATy']['GBs']:.0f} GB/s")
+        _, st, _ = P.solve(lp, P.default_options(tolerance=1e-30, max_iterations=64 * 4, precision=prec,
+                                                 kernel_timing_reps=reps))
+        it = {}
+        for k in ("col", "row"):
+            b, ms = st[f"{k}_kernel_bytes"], st[f"{k}_kernel_ms"]
+            it[f"k_{k}"] = {"us": ms * 1e3, "bytes": b, "GBs": b / (ms * 1e-3) / 1e9,
+                            "frac": b / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+        out["iteration"][name] = it
+        log(f"large-LP iteration {name}: k_col {it['k_col']['us']:.1f} us {it['k_col']['GBs']:.0f} GB/s, "
+            f"k_row {it['k_row']['us']:.1f} us {it['k_row']['GBs']:.0f} GB/s")
+    return out
+
+
 def elapsed():
     return time.perf_counter() - T0
 
@@ -295,6 +339,9 @@ def main():
     ap.add_argument("--calibrate", metavar="JSON", help="run the CPU/GPU crossover sweep and router re-fit only")
     ap.add_argument("--spmv-only", action="store_true",
                     help="print the standalone SpMV rates on the refinery matrix (one JSON line) and exit")
+    ap.add_argument("--large-lp", type=int, metavar="PERIODS",
+                    help="kernel rates on a refinery LP of PERIODS periods (1200: beyond the Infinity Cache); "
+                         "one JSON line, then exit")
     ap.add_argument("--plain-all", metavar="JSON",
                     help="solve every case of the family with plain single-instance GPU PDHG, write JSON, exit")
     args = ap.parse_args()
@@ -310,6 +357,9 @@ def main():
         return
     if args.spmv_only:
         print(json.dumps({"spmv": spmv_rates(P.gen_refinery(SEED_LP))}))
+        return
+    if args.large_lp:
+        print(json.dumps(large_lp_rates(args.large_lp)))
         return
     if args.copy_calibration:
         # A plain device-to-device copy of a known size: its FETCH_SIZE /
