@@ -226,6 +226,23 @@ int pdhg_backend_diff2(pdhg_backend* be, int a, int b, double* out);   /* pdhg.c
  * events on the backend stream; writes the mean kernel time in ms. */
 int pdhg_backend_time_spmv(pdhg_backend* be, int transpose, int reps, double* ms);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The KKT sums of pdhg_backend_kkt over caller-owned
+ * DEVICE arrays, k-interleaved like the backend's vectors, in FP64 (double
+ * arrays) or FP32 (float arrays, precision PDHG_FP32); rfac[m] and cfac[n] are
+ * FP64.  rows3k[3k] and cols3k[3k] are device FP64 outputs, bit-identical to
+ * pdhg_backend_kkt's on the same data.  The work is enqueued on `stream` (a
+ * hipStream_t of the current device; NULL = the null stream) and not waited
+ * for.  It calls no allocating routine: the block partials live in one fixed
+ * device array of the library (2 x 256 blocks x 64 instances x 3 doubles =
+ * 768 KiB per GPU, held until the process ends), so k is limited to 1..64 and
+ * calls that overlap on one GPU race.  Runs on the current device. */
+int pdhg_kkt_noalloc(int m, int n, int k, int precision, const void* axp, const void* rl, const void* ru,
+                     const void* yp, const double* rfac, const void* xp, const void* c, const void* atyp,
+                     const void* lo, const void* up, const double* cfac, double* rows3k, double* cols3k,
+                     void* stream);
+
 /* ---- certifier handoff (src/cert/certifier.h:198-201) ------------------ */
 /* Rigorous Neumaier-Shcherbina lower bound on min c'x valid for any y,
  * outward-rounded FP64 interval arithmetic on the host. */

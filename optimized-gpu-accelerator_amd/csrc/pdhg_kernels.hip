@@ -1694,6 +1694,67 @@ class OpBackend final : public OpBackendBase {
   int *iter_, *mask_;
 };
 
+// ---- KKT sums without allocation ------------------------------------------
+// The block partials of the KKT kernels live in one fixed device array of the
+// library, `scratch`: the row partials in its first half, the column partials
+// in its second, at most kNoAllocBlocks blocks x 64 instances x 3 sums each.
+// Each GPU holds its own copy until the process ends.  Host code never uses
+// the array's name as an address (it is a host-side stand-in there): it uses
+// the per-device address hipGetSymbolAddress returns.
+constexpr int kNoAllocBlocks = 256;
+constexpr int kNoAllocMaxK = 64;
+constexpr size_t kScratchHalf = static_cast<size_t>(kNoAllocBlocks) * kNoAllocMaxK * 3;
+static __device__ __align__(256) double scratch[2 * kScratchHalf];
+
+// out[3b + s] = sum over blocks q = 0 .. g-1 of partial[(b g + q) 3 + s], in
+// that order from 0: the order of the host sum of the op-level backend.
+__global__ void k_sum3_fixed(int g, int k, const double* partial, double* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * k) return;
+  const int b = t / 3, s = t % 3;
+  double acc = 0.0;
+  for (int q = 0; q < g; ++q) acc += partial[(static_cast<size_t>(b) * g + q) * 3 + s];
+  out[t] = acc;
+}
+
+std::mutex gScratchMutex;
+double* gScratchPtr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+double* scratch_on_current_device() {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  PDHG_REQUIRE(dev >= 0 && dev < kMaxDevices, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(gScratchMutex);
+  if (!gScratchPtr[dev]) {
+    void* ptr = nullptr;
+    HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    PDHG_REQUIRE(ptr != nullptr, "no device address for the scratch array");
+    gScratchPtr[dev] = static_cast<double*>(ptr);
+  }
+  return gScratchPtr[dev];
+}
+
+template <typename T>
+void kkt_noalloc(int m, int n, int k, const void* axp, const void* rl, const void* ru, const void* yp,
+                 const double* rfac, const void* xp, const void* c, const void* atyp, const void* lo, const void* up,
+                 const double* cfac, double* rows3k, double* cols3k, hipStream_t s) {
+  double* part = scratch_on_current_device();
+  auto blocks = [](int len) { return std::max(1, std::min(kNoAllocBlocks, (len + kWG - 1) / kWG)); };
+  const dim3 sumGrid((3 * k + 63) / 64), sumBlock(64);
+  const int gr = m > 0 ? blocks(m) : 0, gc = n > 0 ? blocks(n) : 0;
+  if (gr > 0)
+    hipLaunchKernelGGL(k_op_kkt_rows<T>, dim3(gr, k), dim3(kWG), 0, s, m, k, static_cast<const T*>(axp),
+                       static_cast<const T*>(rl), static_cast<const T*>(ru), static_cast<const T*>(yp), rfac, part);
+  hipLaunchKernelGGL(k_sum3_fixed, sumGrid, sumBlock, 0, s, gr, k, part, rows3k);
+  if (gc > 0)
+    hipLaunchKernelGGL(k_op_kkt_cols<T>, dim3(gc, k), dim3(kWG), 0, s, n, k, static_cast<const T*>(xp),
+                       static_cast<const T*>(c), static_cast<const T*>(atyp), static_cast<const T*>(lo),
+                       static_cast<const T*>(up), cfac, part + kScratchHalf);
+  hipLaunchKernelGGL(k_sum3_fixed, sumGrid, sumBlock, 0, s, gc, k, part + kScratchHalf, cols3k);
+  HIP_OK(hipGetLastError());
+}
+
 thread_local std::string tErr;
 
 template <class F>
@@ -1800,6 +1861,30 @@ int pdhg_backend_kkt(pdhg_backend* be, double* rows3k, double* cols3k) { PDHG_BE
 int pdhg_backend_diff2(pdhg_backend* be, int a, int b, double* out) { PDHG_BE_CALL(be->impl->diff2(a, b, out)); }
 int pdhg_backend_time_spmv(pdhg_backend* be, int transpose, int reps, double* ms) {
   PDHG_BE_CALL(*ms = be->impl->timeSpmv(transpose != 0, reps < 1 ? 1 : reps));
+}
+
+int pdhg_kkt_noalloc(int m, int n, int k, int precision, const void* axp, const void* rl, const void* ru,
+                     const void* yp, const double* rfac, const void* xp, const void* c, const void* atyp,
+                     const void* lo, const void* up, const double* cfac, double* rows3k, double* cols3k,
+                     void* stream) {
+  try {
+    using namespace pdhgamd;
+    PDHG_REQUIRE(m >= 0 && n >= 0, "bad sizes");
+    PDHG_REQUIRE(k >= 1 && k <= kNoAllocMaxK, "pdhg_kkt_noalloc: k must be in 1..64 (the scratch array's limit)");
+    PDHG_REQUIRE(precision == PDHG_FP32 || precision == PDHG_FP64, "precision must be FP32 or FP64");
+    PDHG_REQUIRE(rows3k && cols3k, "null output");
+    PDHG_REQUIRE(m == 0 || (axp && rl && ru && yp && rfac), "null row array");
+    PDHG_REQUIRE(n == 0 || (xp && c && atyp && lo && up && cfac), "null column array");
+    auto s = static_cast<hipStream_t>(stream);
+    if (precision == PDHG_FP64)
+      kkt_noalloc<double>(m, n, k, axp, rl, ru, yp, rfac, xp, c, atyp, lo, up, cfac, rows3k, cols3k, s);
+    else
+      kkt_noalloc<float>(m, n, k, axp, rl, ru, yp, rfac, xp, c, atyp, lo, up, cfac, rows3k, cols3k, s);
+    return 0;
+  } catch (const std::exception& e) {
+    pdhg_set_last_error(e.what());
+    return -1;
+  }
 }
 
 }  // extern "C"

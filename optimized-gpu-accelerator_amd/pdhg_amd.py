@@ -82,7 +82,7 @@ EXPORTS = [
     "pdhg_backend_create", "pdhg_backend_destroy", "pdhg_backend_set", "pdhg_backend_get",
     "pdhg_backend_spmv", "pdhg_backend_spmvT", "pdhg_backend_set_steps", "pdhg_backend_primal_step",
     "pdhg_backend_dual_step", "pdhg_backend_halpern", "pdhg_backend_restart", "pdhg_backend_kkt",
-    "pdhg_backend_diff2", "pdhg_backend_time_spmv",
+    "pdhg_backend_diff2", "pdhg_backend_time_spmv", "pdhg_kkt_noalloc",
     "pdhg_safe_dual_bound", "pdhg_certify", "pdhg_certify_batch",
     "pdhg_gen_random_lp", "pdhg_gen_refinery", "pdhg_gen_family", "pdhg_family_costs",
     "pdhg_gen_calibration", "pdhg_read_mps", "pdhg_model_view", "pdhg_model_free",
@@ -140,6 +140,8 @@ def lib() -> C.CDLL:
         L.pdhg_backend_kkt.argtypes = [C.c_void_p, _dp, _dp]
         L.pdhg_backend_diff2.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
         L.pdhg_backend_time_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.pdhg_kkt_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4 + [C.c_void_p] + \
+            [C.c_void_p] * 5 + [C.c_void_p] * 3 + [C.c_void_p]
         L.pdhg_pod_basis.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.pdhg_decode_bf16.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, C.c_int, _dp]
         L.pdhg_mlp_create.restype = C.c_void_p
