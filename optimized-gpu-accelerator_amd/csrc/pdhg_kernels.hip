@@ -489,35 +489,78 @@ __global__ __launch_bounds__(kWG) void k_spmv(RowPlan plan, const int* __restric
     if (rows[rr] >= 0 && s.sub == 0) vst<SPL, LOGB>(out, rows[rr], b, accs[rr]);
 }
 
+// Vector layout of the op-level kernels: element (i, b) of rows of `ld`
+// lanes at i*ld + b, or, for FP64 with `split`, the run layout of the
+// batched kernels (the low words of row i's ld lanes, then their high words).
+struct OpLay {
+  int ld;
+  int split;
+};
+
 template <typename T>
-__global__ void k_op_primal(long long len, int k, const T* x, const T* aty, const T* c, const T* lo, const T* up,
+__device__ inline T oget(const T* __restrict__ p, size_t i, int b, OpLay L) {
+  if constexpr (sizeof(T) == 8) {
+    if (L.split) {
+      const unsigned* u = reinterpret_cast<const unsigned*>(p) + i * 2 * L.ld;
+      return __builtin_bit_cast(T, (static_cast<unsigned long long>(u[L.ld + b]) << 32) | u[b]);
+    }
+  }
+  return p[i * L.ld + b];
+}
+
+template <typename T>
+__device__ inline void oput(T* __restrict__ p, size_t i, int b, OpLay L, T v) {
+  if constexpr (sizeof(T) == 8) {
+    if (L.split) {
+      unsigned* u = reinterpret_cast<unsigned*>(p) + i * 2 * L.ld;
+      const unsigned long long bits = __builtin_bit_cast(unsigned long long, v);
+      u[b] = static_cast<unsigned>(bits);
+      u[L.ld + b] = static_cast<unsigned>(bits >> 32);
+      return;
+    }
+  }
+  p[i * L.ld + b] = v;
+}
+
+// Elementwise ops over rows x ld elements (element (i, b) with b = t mod ld).
+template <typename T>
+__global__ void k_op_primal(long long len, OpLay L, const T* x, const T* aty, const T* c, const T* lo, const T* up,
                             const double* tau, T* xp) {
   for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < len;
        t += static_cast<long long>(gridDim.x) * blockDim.x) {
-    const T tv = static_cast<T>(tau[t % k]);
-    xp[t] = clamp_to<T>(x[t] - tv * (c[t] - aty[t]), lo[t], up[t]);
+    const size_t i = static_cast<size_t>(t / L.ld);
+    const int b = static_cast<int>(t % L.ld);
+    const T tv = static_cast<T>(tau[b]);
+    oput(xp, i, b, L,
+         clamp_to<T>(oget(x, i, b, L) - tv * (oget(c, i, b, L) - oget(aty, i, b, L)), oget(lo, i, b, L),
+                     oget(up, i, b, L)));
   }
 }
 
 template <typename T>
-__global__ void k_op_dual(long long len, int k, const T* y, const T* axp, const T* ax, const T* rl, const T* ru,
+__global__ void k_op_dual(long long len, OpLay L, const T* y, const T* axp, const T* ax, const T* rl, const T* ru,
                           const double* sigma, T* yp) {
   for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < len;
        t += static_cast<long long>(gridDim.x) * blockDim.x) {
-    const T s = static_cast<T>(sigma[t % k]);
-    const T v = y[t] - s * (static_cast<T>(2) * axp[t] - ax[t]);
-    yp[t] = v + s * clamp_to<T>(-v / s, rl[t], ru[t]);
+    const size_t i = static_cast<size_t>(t / L.ld);
+    const int b = static_cast<int>(t % L.ld);
+    const T s = static_cast<T>(sigma[b]);
+    const T v = oget(y, i, b, L) - s * (static_cast<T>(2) * oget(axp, i, b, L) - oget(ax, i, b, L));
+    oput(yp, i, b, L, v + s * clamp_to<T>(-v / s, oget(rl, i, b, L), oget(ru, i, b, L)));
   }
 }
 
 template <typename T>
-__global__ void k_op_halpern(long long len, int k, const int* iter, double rho, T* z, const T* zp, const T* z0) {
+__global__ void k_op_halpern(long long len, OpLay L, const int* iter, double rho, T* z, const T* zp, const T* z0) {
   for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < len;
        t += static_cast<long long>(gridDim.x) * blockDim.x) {
-    const double kk = iter[t % k];
+    const size_t i = static_cast<size_t>(t / L.ld);
+    const int b = static_cast<int>(t % L.ld);
+    const double kk = iter[b];
     const double lam = (kk + 1.0) / (kk + 2.0);
-    z[t] = static_cast<T>(lam) * (static_cast<T>(1.0 + rho) * zp[t] - static_cast<T>(rho) * z[t]) +
-           static_cast<T>(1.0 - lam) * z0[t];
+    oput(z, i, b, L,
+         static_cast<T>(lam) * (static_cast<T>(1.0 + rho) * oget(zp, i, b, L) - static_cast<T>(rho) * oget(z, i, b, L)) +
+             static_cast<T>(1.0 - lam) * oget(z0, i, b, L));
   }
 }
 
@@ -532,10 +575,17 @@ __global__ void k_op_advance(int k, int* iter, const int* mask, int reset) {
 }
 
 template <typename T>
-__global__ void k_op_restart(long long len, int k, const int* mask, const T* src, T* d1, T* d2) {
+__global__ void k_op_restart(long long len, OpLay L, const int* mask, const T* src, T* d1, T* d2) {
   for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < len;
-       t += static_cast<long long>(gridDim.x) * blockDim.x)
-    if (mask[t % k]) d1[t] = d2[t] = src[t];
+       t += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const size_t i = static_cast<size_t>(t / L.ld);
+    const int b = static_cast<int>(t % L.ld);
+    if (mask[b]) {
+      const T v = oget(src, i, b, L);
+      oput(d1, i, b, L, v);
+      oput(d2, i, b, L, v);
+    }
+  }
 }
 
 // Partial sums partial[(b * g + blk) * 3 + s] over a grid (g, k), as the
@@ -559,13 +609,12 @@ __device__ inline void op_block_sum3(double a, double bq, double cq, double* out
 }
 
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_op_kkt_rows(int m, int k, const T* axp, const T* rl, const T* ru,
+__global__ __launch_bounds__(kWG) void k_op_kkt_rows(int m, OpLay Y, const T* axp, const T* rl, const T* ru,
                                                      const T* yp, const double* rfac, double* partial) {
   const int b = blockIdx.y;
   double s0 = 0, s1 = 0, s2 = 0;
   for (int i = blockIdx.x * kWG + threadIdx.x; i < m; i += gridDim.x * kWG) {
-    const size_t t = static_cast<size_t>(i) * k + b;
-    const double a = axp[t], L = rl[t], U = ru[t], y = yp[t];
+    const double a = oget(axp, i, b, Y), L = oget(rl, i, b, Y), U = oget(ru, i, b, Y), y = oget(yp, i, b, Y);
     const double res = (fmax(L - a, 0.0) + fmax(a - U, 0.0)) * rfac[i];
     s0 += res * res;
     if (y > 0 && L > -1e300) s1 += y * L;
@@ -576,15 +625,14 @@ __global__ __launch_bounds__(kWG) void k_op_kkt_rows(int m, int k, const T* axp,
 }
 
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_op_kkt_cols(int n, int k, const T* xp, const T* c, const T* atyp,
+__global__ __launch_bounds__(kWG) void k_op_kkt_cols(int n, OpLay X, const T* xp, const T* c, const T* atyp,
                                                      const T* lo, const T* up, const double* cfac,
                                                      double* partial) {
   const int b = blockIdx.y;
   double s0 = 0, s1 = 0, s2 = 0;
   for (int j = blockIdx.x * kWG + threadIdx.x; j < n; j += gridDim.x * kWG) {
-    const size_t t = static_cast<size_t>(j) * k + b;
-    const double cj = c[t], l = lo[t], u = up[t];
-    const double d = cj - static_cast<double>(atyp[t]);
+    const double cj = oget(c, j, b, X), l = oget(lo, j, b, X), u = oget(up, j, b, X);
+    const double d = cj - static_cast<double>(oget(atyp, j, b, X));
     const bool hl = l > -1e300, hu = u < 1e300;
     double res = 0;
     if (!hl && !hu) res = fabs(d);
@@ -592,7 +640,7 @@ __global__ __launch_bounds__(kWG) void k_op_kkt_cols(int n, int k, const T* xp, 
     else if (!hu) res = fmax(-d, 0.0);
     res *= cfac[j];
     s0 += res * res;
-    s1 += cj * static_cast<double>(xp[t]);
+    s1 += cj * static_cast<double>(oget(xp, j, b, X));
     if (d > 0 && hl) s2 += d * l;
     else if (d < 0 && hu) s2 += d * u;
   }
@@ -600,12 +648,11 @@ __global__ __launch_bounds__(kWG) void k_op_kkt_cols(int n, int k, const T* xp, 
 }
 
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_op_diff2(int len, int k, const T* a, const T* bv, double* partial) {
+__global__ __launch_bounds__(kWG) void k_op_diff2(int len, OpLay L, const T* a, const T* bv, double* partial) {
   const int b = blockIdx.y;
   double s = 0;
   for (int i = blockIdx.x * kWG + threadIdx.x; i < len; i += gridDim.x * kWG) {
-    const size_t t = static_cast<size_t>(i) * k + b;
-    const double d = static_cast<double>(a[t]) - static_cast<double>(bv[t]);
+    const double d = static_cast<double>(oget(a, i, b, L)) - static_cast<double>(oget(bv, i, b, L));
     s += d * d;
   }
   op_block_sum3(s, 0.0, 0.0, partial + (static_cast<size_t>(b) * gridDim.x + blockIdx.x) * 3);
@@ -1495,20 +1542,15 @@ class OpBackend final : public OpBackendBase {
     PDHG_REQUIRE(k >= 1 && k <= 64, "backend batch width must be in 1..64");
     logB_ = ilog2(k);
     B_ = 1 << logB_;
+    split_ = sizeof(T) == 8 && logB_ >= 4;  // the batched kernels' layout (kSplit)
+    lay_ = OpLay{B_, split_ ? 1 : 0};
     kern_ = kern_table<T>(logB_);
     aPtr_ = up(A.ptr);
-    aIdx_ = up(A.idx);
+    aIdx_ = up(csr_padded(A.idx));  // whole-chunk reads past a row stay in bounds
     tPtr_ = up(AT.ptr);
-    tIdx_ = up(AT.idx);
-    aVal_ = upT(A.val);
-    tVal_ = upT(AT.val);
-    MapSetup ms;
-    rowPlan_ = build_plan(A, logB_, ms);
-    colPlan_ = build_plan(AT, logB_, ms);
-    for (PlanHost* h : {&rowPlan_, &colPlan_}) {
-      if (!h->perm.empty()) h->dPerm = up(h->perm);
-      h->plan.perm = h->dPerm;
-    }
+    tIdx_ = up(csr_padded(AT.idx));
+    aVal_ = upT(csr_padded(A.val));
+    tVal_ = upT(csr_padded(AT.val));
     rfac_ = buf_.alloc<double>(m);
     cfac_ = buf_.alloc<double>(n);
     put(rfac_, rfac, m);
@@ -1528,6 +1570,10 @@ class OpBackend final : public OpBackendBase {
     put(tau_, one.data(), B_);
     put(sigma_, one.data(), B_);
     partial_ = buf_.alloc<double>(static_cast<size_t>(kRed) * k * 3);
+    // Row mapping of the two SpMVs: the fastest candidate of the batched
+    // engine's autotuner list (LDS tiles aside) for this matrix and width.
+    rowPlan_ = tune_spmv(A, aPtr_, aIdx_, aVal_, vec_[PDHG_V_X], vec_[PDHG_V_AX]);
+    colPlan_ = tune_spmv(AT, tPtr_, tIdx_, tVal_, vec_[PDHG_V_Y], vec_[PDHG_V_ATY]);
     HIP_OK(hipStreamSynchronize(stream_));
   }
   ~OpBackend() override {
@@ -1548,6 +1594,7 @@ class OpBackend final : public OpBackendBase {
     const int rows = rows_of(v);
     for (int i = 0; i < rows; ++i)
       for (int b = 0; b < k_; ++b) t[static_cast<size_t>(i) * B_ + b] = static_cast<T>(d[static_cast<size_t>(i) * k_ + b]);
+    if (split_) to_split(t, rows);
     put(vec_[v], t.data(), t.size());
   }
   void get(int v, double* d) override {
@@ -1557,6 +1604,7 @@ class OpBackend final : public OpBackendBase {
     HIP_OK(hipMemcpyAsync(t.data(), vec_[v], t.size() * sizeof(T), hipMemcpyDeviceToHost, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
     const int rows = rows_of(v);
+    if (split_) from_split(t, rows);
     for (int i = 0; i < rows; ++i)
       for (int b = 0; b < k_; ++b) d[static_cast<size_t>(i) * k_ + b] = t[static_cast<size_t>(i) * B_ + b];
   }
@@ -1564,8 +1612,9 @@ class OpBackend final : public OpBackendBase {
     check_vec(in);
     check_vec(out);
     const PlanHost& p = transpose ? colPlan_ : rowPlan_;
-    hipLaunchKernelGGL(kern_.spmvStd, dim3(p.blocks), dim3(kWG), 0, stream_, p.plan, transpose ? tPtr_ : aPtr_,
-                       transpose ? tIdx_ : aIdx_, transpose ? tVal_ : aVal_, vec_[in], vec_[out]);
+    hipLaunchKernelGGL(kern_.spmv[p.plan.rpw - 1], dim3(p.blocks), dim3(kWG), 0, stream_, p.plan,
+                       transpose ? tPtr_ : aPtr_, transpose ? tIdx_ : aIdx_, transpose ? tVal_ : aVal_, vec_[in],
+                       vec_[out]);
     HIP_OK(hipGetLastError());
   }
   void setSteps(const double* tau, const double* sigma) override {
@@ -1575,20 +1624,20 @@ class OpBackend final : public OpBackendBase {
   }
   void primal() override {
     const long long len = static_cast<long long>(n_) * B_;
-    hipLaunchKernelGGL(k_op_primal<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, B_, vec_[PDHG_V_X],
+    hipLaunchKernelGGL(k_op_primal<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, lay_, vec_[PDHG_V_X],
                        vec_[PDHG_V_ATY], vec_[PDHG_V_C], vec_[PDHG_V_LO], vec_[PDHG_V_UP], tau_, vec_[PDHG_V_XP]);
     HIP_OK(hipGetLastError());
   }
   void dual() override {
     const long long len = static_cast<long long>(m_) * B_;
-    hipLaunchKernelGGL(k_op_dual<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, B_, vec_[PDHG_V_Y],
+    hipLaunchKernelGGL(k_op_dual<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, lay_, vec_[PDHG_V_Y],
                        vec_[PDHG_V_AXP], vec_[PDHG_V_AX], vec_[PDHG_V_RL], vec_[PDHG_V_RU], sigma_, vec_[PDHG_V_YP]);
     HIP_OK(hipGetLastError());
   }
   void halpern(double rho) override {
     auto one = [&](int rows, int z, int zp, int z0) {
       const long long len = static_cast<long long>(rows) * B_;
-      hipLaunchKernelGGL(k_op_halpern<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, B_, iter_, rho, vec_[z],
+      hipLaunchKernelGGL(k_op_halpern<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, lay_, iter_, rho, vec_[z],
                          vec_[zp], vec_[z0]);
     };
     one(n_, PDHG_V_X, PDHG_V_XP, PDHG_V_X0);
@@ -1602,7 +1651,7 @@ class OpBackend final : public OpBackendBase {
     put(mask_, mask, k_);
     auto cp = [&](int rows, int src, int d1, int d2) {
       const long long len = static_cast<long long>(rows) * B_;
-      hipLaunchKernelGGL(k_op_restart<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, B_, mask_, vec_[src],
+      hipLaunchKernelGGL(k_op_restart<T>, dim3(grid(len)), dim3(kWG), 0, stream_, len, lay_, mask_, vec_[src],
                          vec_[d1], vec_[d2]);
     };
     cp(n_, PDHG_V_XP, PDHG_V_X, PDHG_V_X0);
@@ -1617,12 +1666,12 @@ class OpBackend final : public OpBackendBase {
     std::vector<double> host;
     if (m_ > 0) {
       const int g = red_blocks(m_);
-      hipLaunchKernelGGL(k_op_kkt_rows<T>, dim3(g, k_), dim3(kWG), 0, stream_, m_, B_, vec_[PDHG_V_AXP],
+      hipLaunchKernelGGL(k_op_kkt_rows<T>, dim3(g, k_), dim3(kWG), 0, stream_, m_, lay_, vec_[PDHG_V_AXP],
                          vec_[PDHG_V_RL], vec_[PDHG_V_RU], vec_[PDHG_V_YP], rfac_, partial_);
       fetch_sum3(g, r3);
     }
     const int g = red_blocks(n_);
-    hipLaunchKernelGGL(k_op_kkt_cols<T>, dim3(g, k_), dim3(kWG), 0, stream_, n_, B_, vec_[PDHG_V_XP],
+    hipLaunchKernelGGL(k_op_kkt_cols<T>, dim3(g, k_), dim3(kWG), 0, stream_, n_, lay_, vec_[PDHG_V_XP],
                        vec_[PDHG_V_C], vec_[PDHG_V_ATYP], vec_[PDHG_V_LO], vec_[PDHG_V_UP], cfac_, partial_);
     fetch_sum3(g, c3);
   }
@@ -1633,7 +1682,7 @@ class OpBackend final : public OpBackendBase {
     std::fill(out, out + k_, 0.0);
     if (len == 0) return;
     const int g = red_blocks(len);
-    hipLaunchKernelGGL(k_op_diff2<T>, dim3(g, k_), dim3(kWG), 0, stream_, len, B_, vec_[a], vec_[b], partial_);
+    hipLaunchKernelGGL(k_op_diff2<T>, dim3(g, k_), dim3(kWG), 0, stream_, len, lay_, vec_[a], vec_[b], partial_);
     std::vector<double> r3(3 * k_, 0.0);
     fetch_sum3(g, r3.data());
     for (int q = 0; q < k_; ++q) out[q] = r3[3 * q];
@@ -1674,6 +1723,72 @@ class OpBackend final : public OpBackendBase {
     put(d, v.data(), v.size());
     return d;
   }
+  // B-strided FP64 rows <-> the split layout (low words of a row's B lanes,
+  // then their high words).
+  void to_split(std::vector<T>& t, int rows) const {
+    std::vector<unsigned> u(t.size() * 2);
+    for (size_t i = 0; i < static_cast<size_t>(rows); ++i)
+      for (int b = 0; b < B_; ++b) {
+        unsigned long long bits;
+        std::memcpy(&bits, &t[i * B_ + b], sizeof(bits));
+        u[i * 2 * B_ + b] = static_cast<unsigned>(bits);
+        u[i * 2 * B_ + B_ + b] = static_cast<unsigned>(bits >> 32);
+      }
+    std::memcpy(t.data(), u.data(), t.size() * sizeof(T));
+  }
+  void from_split(std::vector<T>& t, int rows) const {
+    std::vector<unsigned> u(t.size() * 2);
+    std::memcpy(u.data(), t.data(), t.size() * sizeof(T));
+    for (size_t i = 0; i < static_cast<size_t>(rows); ++i)
+      for (int b = 0; b < B_; ++b) {
+        const unsigned long long bits =
+            (static_cast<unsigned long long>(u[i * 2 * B_ + B_ + b]) << 32) | u[i * 2 * B_ + b];
+        std::memcpy(&t[i * B_ + b], &bits, sizeof(bits));
+      }
+  }
+  // Times every candidate mapping of one SpMV (8 launches after a warm one,
+  // HIP events on the backend's stream, zero vectors) and keeps the fastest.
+  PlanHost tune_spmv(const Csr& M, const int* ptr, const int* idx, const T* val, const T* in, T* out) {
+    std::vector<MapSetup> cand;
+    for (bool xcd : {false, true}) {
+      if (B_ == 64) {
+        for (int rpw : {1, 2}) cand.push_back(MapSetup{4, 1, xcd, false, rpw});
+        continue;
+      }
+      for (int per : {2, 4, 8, 16})
+        for (int map : {1, 2}) cand.push_back(MapSetup{per, map, xcd});
+      if (B_ == 16 || B_ == 32)
+        for (int map : {1, 2}) cand.push_back(MapSetup{4, map, xcd, true});
+    }
+    int* scratch = buf_.alloc<int>(static_cast<size_t>(std::max(1, M.rows)));
+    double bestMs = 0;
+    size_t best = 0;
+    for (size_t c = 0; c < cand.size(); ++c) {
+      PlanHost h = build_plan(M, logB_, cand[c]);
+      if (!h.perm.empty()) put(scratch, h.perm.data(), h.perm.size());
+      h.plan.perm = h.perm.empty() ? nullptr : scratch;
+      auto launch = [&] {
+        hipLaunchKernelGGL(kern_.spmv[h.plan.rpw - 1], dim3(h.blocks), dim3(kWG), 0, stream_, h.plan, ptr, idx, val,
+                           in, out);
+      };
+      launch();
+      HIP_OK(hipEventRecord(e0_, stream_));
+      for (int r = 0; r < 8; ++r) launch();
+      HIP_OK(hipEventRecord(e1_, stream_));
+      HIP_OK(hipEventSynchronize(e1_));
+      HIP_OK(hipGetLastError());
+      float ms = 0;
+      HIP_OK(hipEventElapsedTime(&ms, e0_, e1_));
+      if (c == 0 || ms < bestMs) {
+        bestMs = ms;
+        best = c;
+      }
+    }
+    PlanHost h = build_plan(M, logB_, cand[best]);
+    if (!h.perm.empty()) h.dPerm = up(h.perm);
+    h.plan.perm = h.dPerm;
+    return h;
+  }
   T* upT(const std::vector<double>& v) {
     std::vector<T> t(v.begin(), v.end());
     T* d = buf_.alloc<T>(v.size());
@@ -1682,6 +1797,8 @@ class OpBackend final : public OpBackendBase {
   }
 
   int m_, n_, k_, logB_ = 0, B_ = 1;
+  bool split_ = false;
+  OpLay lay_{1, 0};
   hipStream_t stream_ = nullptr;
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
   Buffers buf_;
@@ -1744,11 +1861,11 @@ void kkt_noalloc(int m, int n, int k, const void* axp, const void* rl, const voi
   const dim3 sumGrid((3 * k + 63) / 64), sumBlock(64);
   const int gr = m > 0 ? blocks(m) : 0, gc = n > 0 ? blocks(n) : 0;
   if (gr > 0)
-    hipLaunchKernelGGL(k_op_kkt_rows<T>, dim3(gr, k), dim3(kWG), 0, s, m, k, static_cast<const T*>(axp),
+    hipLaunchKernelGGL(k_op_kkt_rows<T>, dim3(gr, k), dim3(kWG), 0, s, m, OpLay{k, 0}, static_cast<const T*>(axp),
                        static_cast<const T*>(rl), static_cast<const T*>(ru), static_cast<const T*>(yp), rfac, part);
   hipLaunchKernelGGL(k_sum3_fixed, sumGrid, sumBlock, 0, s, gr, k, part, rows3k);
   if (gc > 0)
-    hipLaunchKernelGGL(k_op_kkt_cols<T>, dim3(gc, k), dim3(kWG), 0, s, n, k, static_cast<const T*>(xp),
+    hipLaunchKernelGGL(k_op_kkt_cols<T>, dim3(gc, k), dim3(kWG), 0, s, n, OpLay{k, 0}, static_cast<const T*>(xp),
                        static_cast<const T*>(c), static_cast<const T*>(atyp), static_cast<const T*>(lo),
                        static_cast<const T*>(up), cfac, part + kScratchHalf);
   hipLaunchKernelGGL(k_sum3_fixed, sumGrid, sumBlock, 0, s, gc, k, part + kScratchHalf, cols3k);

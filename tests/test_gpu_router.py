@@ -13,6 +13,7 @@ test_committed_router_model_drives_the_reference_router loads that file into
 the reference's router through --router-model and checks its choices."""
 import json
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -38,21 +39,29 @@ def generated(tmp):
     for periods, streams in ((2, 12), (4, 20), (12, 40)):
         lp = P.gen_refinery(3, periods=periods, streams=streams)
         out[f"refinery_{periods}x{streams}"] = write_mps(lp, tmp / f"ref_{periods}.mps.gz", "REFINERY")
-    for s in (1000, 4000, 16000, 64000):
+    for s in (1000, 4000):
         lp = P.gen_random_lp(1, s, 2 * s, 4.0, 0.0)
         out[f"random_{s}"] = write_mps(lp, tmp / f"rand_{s}.mps.gz", "RANDLP")
     out["plan_6x4"] = os.path.join(I.GOLDEN, "plan_6x4.mps.gz")
     return out
 
 
-def run(path, engine, tmp, extra=()):
+def run(path, engine, tmp):
+    """One solve by `engine` under the time limit; the PDHG engines without the
+    crossover (the router adds that polish to their prediction itself,
+    router.cpp:181-183).  A run that outlives the limit by far is stopped and
+    counts as unproven."""
     js = tmp / "r.json"
     if js.exists():
         js.unlink()
-    rc, out, err = I.cli(str(path), "--algo", engine, "--time", str(LIMIT), "--json", str(js), "--log", "0", *extra,
-                         timeout=LIMIT * 6 + 60)
-    j = json.loads(js.read_text())
-    return j, rc
+    extra = ["--no-crossover"] if engine.startswith("pdhg") else []
+    print(f"  {os.path.basename(str(path))} {engine}", flush=True)
+    try:
+        rc, out, err = I.cli(str(path), "--algo", engine, "--time", str(LIMIT), "--json", str(js), "--log", "0",
+                             *extra, timeout=LIMIT * 3 + 30)
+    except subprocess.TimeoutExpired:
+        return None
+    return json.loads(js.read_text()) if js.exists() else None
 
 
 def test_router_sweep(gpu, tmp_path):
@@ -68,11 +77,17 @@ def test_router_sweep(gpu, tmp_path):
     for n, path in inst.items():
         rec = {"instance": n, "family": fam[n], "times": {}, "status": {}}
         for e in R.ENGINES:
-            j, rc = run(path, e, tmp_path)
+            j = run(path, e, tmp_path)
+            if j is None:
+                rec["status"][e] = "STOPPED"
+                rec["times"][e] = R.censored_time(None, False, LIMIT)
+                continue
             rec["features"] = j["telemetry"]["router"]["features"]
             rec["status"][e] = j["status"]
             rec["times"][e] = R.censored_time(j.get("seconds"), j["status"] in PROVEN, LIMIT)
             rec.setdefault("engine_label", {})[e] = j.get("engine")
+        if "features" not in rec:
+            continue
         print(n, json.dumps(rec["times"]), flush=True)
         records.append(rec)
     fit = R.fit_router(records)
