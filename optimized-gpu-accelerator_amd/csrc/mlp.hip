@@ -2,8 +2,8 @@
 // parameter theta to coefficients on a POD basis of PDHG solutions, and the
 // basis decode z = mean + U a gives the starting point of a warm-started
 // solve.  Forward, backward, Adam and decode are hand-written HIP for gfx950:
-// every matrix product runs on BF16 MFMA tiles (v_mfma_f32_16x16x16_bf16)
-// with FP32 accumulation; master weights, activations and optimiser state
+// every matrix product runs on gfx950 BF16 MFMA tiles
+// (v_mfma_f32_16x16x32_bf16) with FP32 accumulation; master weights, activations and optimiser state
 // are FP32.  No reference counterpart: the reference has no warm start
 // (x0, y0 are hard-wired, src/pdhg/pdhg.cpp:509,519).
 #include <hip/hip_runtime.h>
@@ -29,15 +29,16 @@ int pod_basis(const double* Z, int k, int d, int rank, double* mean, double* bas
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // FP32 -> BF16 with round-to-nearest-even on the bit pattern (NaN kept quiet).
-__device__ inline short to_bf16(float f) {
+__device__ inline __bf16 to_bf16(float f) {
   unsigned u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return static_cast<short>((u >> 16) | 0x40);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu))
+    return __builtin_bit_cast(__bf16, static_cast<unsigned short>((u >> 16) | 0x40));
   u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<short>(u >> 16);
+  return __builtin_bit_cast(__bf16, static_cast<unsigned short>(u >> 16));
 }
 
 __device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
@@ -60,27 +61,28 @@ struct Gemm {
   int act;              // 1: GELU
 };
 
-// One wavefront per 16 x 16 output tile; K consumed 16 at a time by one MFMA.
-// Lane l feeds A rows (l % 16) / B columns (l % 16) at k = 4 (l / 16) + i and
-// receives C rows 4 (l / 16) + i, column l % 16.  Out-of-range operands load
-// as zero, so any M, N, K works.
+// One wavefront per 16 x 16 output tile; K consumed 32 at a time by one
+// gfx950 MFMA (v_mfma_f32_16x16x32_bf16).  Lane l feeds A row (l % 16) and
+// B column (l % 16) at k = 8 (l / 16) + i, i < 8, and receives C rows
+// 4 (l / 16) + i, column l % 16.  Out-of-range operands load as zero, so any
+// M, N, K works.
 __global__ __launch_bounds__(64) void k_gemm_bf16(Gemm g) {
   const int lane = threadIdx.x;
   const int n0 = blockIdx.x * 16, m0 = blockIdx.y * 16;
   const int r = lane & 15, q = lane >> 4;
   floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < g.K; k0 += 16) {
-    bf16x4 a, b;
+  for (int k0 = 0; k0 < g.K; k0 += 32) {
+    bf16x8 a, b;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int k = k0 + 4 * q + i;
+    for (int i = 0; i < 8; ++i) {
+      const int k = k0 + 8 * q + i;
       const int m = m0 + r, n = n0 + r;
       const float av = (m < g.M && k < g.K) ? g.A[m * g.sam + k * g.sak] : 0.f;
       const float bv = (n < g.N && k < g.K) ? g.B[k * g.sbk + n * g.sbn] : 0.f;
       a[i] = to_bf16(av);
       b[i] = to_bf16(bv);
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
