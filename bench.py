@@ -177,49 +177,6 @@ def cpu_baseline(lp, costs, gpu_iters, seconds, threads):
     }
 
 
-def calibrate(threads, out_path):
-    """The reference's CPU/GPU crossover sweep (calibrateBandwidth,
-    src/parametric/parametric.cpp:537-590) on this machine: seconds per
-    iteration of the reference's own CPU PDHG (oracle/_ref, `threads` host
-    threads) and of this engine (FP64 and FP32 iterates) on the six sweep LPs,
-    then the router re-fit (router_fit.refit) for the engine's default
-    precision."""
-    import router_fit as RF
-
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-
-    if not O.ref_available():
-        raise SystemExit("calibration needs oracle/_ref (the reference CPU PDHG built from its sources)")
-    O.ref_set_threads(threads)
-    rows = []
-    for s in RF.SIZES:
-        lp = RF.sweep_lp(s)
-        r = {"rows": lp.m, "cols": lp.n, "nnz": lp.nnz}
-        ref, _, _ = O.ref_solve(lp, tol=1e-30, max_iter=RF.ITERATIONS)
-        per = ref["solve_seconds"] / max(1, ref["iterations"])
-        r["cpu"] = {"seconds_per_iteration": per, "effective_GBs": RF.effective_gbs(lp, per),
-                    "setup_seconds": ref["setup_seconds"]}
-        r["gpu_fp64"] = RF.gpu_timing(lp, P.FP64)
-        r["gpu_fp32"] = RF.gpu_timing(lp, P.FP32)
-        rows.append(r)
-        log(f"calibrate s={s}: nnz {lp.nnz}, cpu {per * 1e6:.1f} us/it, gpu fp64 "
-            f"{r['gpu_fp64']['seconds_per_iteration'] * 1e6:.1f} us/it, fp32 "
-            f"{r['gpu_fp32']['seconds_per_iteration'] * 1e6:.1f} us/it")
-    out = {"gpu": P.gpu_info(), "cpu_threads": threads, "host_nproc": os.cpu_count(),
-           "iterations_per_point": RF.ITERATIONS, "pdhg_iteration_timing": rows,
-           "fp64": RF.refit(rows, "gpu_fp64"), "fp32": RF.refit(rows, "gpu_fp32")}
-    out["router_model"] = {"weights": out["fp64"]["weights"]}  # the engine's default precision (FP64)
-    with open(out_path, "w") as f:
-        json.dump(out, f, indent=1)
-    log(f"crossover nnz (measured): fp64 {out['fp64']['crossover_nnz_measured']}, "
-        f"fp32 {out['fp32']['crossover_nnz_measured']}; written to {out_path}")
-    print(json.dumps({"calibration": out_path, "fp64": out["fp64"]["ratio_fit"],
-                      "fp32": out["fp32"]["ratio_fit"],
-                      "crossover_nnz_measured": {"fp64": out["fp64"]["crossover_nnz_measured"],
-                                                 "fp32": out["fp32"]["crossover_nnz_measured"]}}))
-
-
 def large_lp_rates(periods, reps=50):
     """Kernels on a refinery LP too large for the 256 MB Infinity Cache
     (`periods` periods of the staircase: 1200 gives 55.6M nonzeros, 445 MB of
@@ -304,6 +261,7 @@ def plain_gpu(lp, costs, opts, sample):
         c = P.certify_batch(lp, costs[b:b + 1], r[0].x[None], r[0].y[None], np.array([r[0].objective]))
         ok += int(r[0].status == P.OPTIMAL and c[0, 3] > 0)
         iters.append(int(r[0].iterations))
+        log(f"plain case {b}: {r[0].iterations} iterations, status {r[0].status}, {time.perf_counter() - t:.1f} s")
     el = time.perf_counter() - t
     return {"solves_per_s": ok / el if sample else None, "cases": sample, "certified": ok, "seconds": el,
             "iterations": iters}
@@ -337,7 +295,6 @@ def main():
                     help="cap per instance (the reference default, src/pdhg/pdhg.h:34); unconverged = unsolved")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device copy of MIB MiB (units of the HBM traffic counters)")
-    ap.add_argument("--calibrate", metavar="JSON", help="run the CPU/GPU crossover sweep and router re-fit only")
     ap.add_argument("--spmv-only", action="store_true",
                     help="print the standalone SpMV rates on the refinery matrix (one JSON line) and exit")
     ap.add_argument("--large-lp", type=int, metavar="PERIODS",
@@ -353,15 +310,6 @@ def main():
     info = P.gpu_info()
     if not info["available"]:
         raise SystemExit(f"no usable gfx950 device: {info}")
-    if args.calibrate:
-        calibrate(args.cpu_threads, args.calibrate)
-        return
-    if args.spmv_only:
-        print(json.dumps({"spmv": spmv_rates(P.gen_refinery(SEED_LP))}))
-        return
-    if args.large_lp:
-        print(json.dumps(large_lp_rates(args.large_lp)))
-        return
     if args.copy_calibration:
         # A plain device-to-device copy of a known size: its FETCH_SIZE /
         # WRITE_SIZE counter reading calibrates the units of the HBM-traffic
@@ -379,6 +327,12 @@ def main():
         hip.hipFree(src)
         hip.hipFree(dst)
         log(f"calibration copy: {args.copy_calibration} MiB device to device")
+    if args.spmv_only:
+        print(json.dumps({"spmv": spmv_rates(P.gen_refinery(SEED_LP))}))
+        return
+    if args.large_lp:
+        print(json.dumps(large_lp_rates(args.large_lp)))
+        return
     prec = {"fp64": P.FP64, "fp32": P.FP32, "fp32auto": P.FP32_AUTO}[args.precision]
     lp = P.gen_refinery(SEED_LP)
     K = args.cases

@@ -193,3 +193,27 @@ def fit_router(records: list[dict], lam: float = 1e-2, folds: int = 5, seed: int
     return {"weights": W, "features": FEATURES, "instances": len(names), "engines": engines,
             "validation": {"router_held_out": regret(chosen) if chosen else None,
                            **{f"always_{e}": regret({n: e for n in names}) for e in engines}}}
+
+
+def crossover_from_sweep(sweep: dict) -> dict:
+    """The CPU/GPU PDHG crossover re-fit from the output of the reference's
+    own `pramana calibrate` verb run over this engine (calibrateBandwidth,
+    src/parametric/parametric.cpp:537-590: rows with "nnz", "cpu" and "gpu"
+    seconds per iteration)."""
+    rows = [r for r in sweep["pdhg_iteration_timing"] if "gpu" in r and "cpu" in r]
+    out = refit(rows, "gpu")
+    out["gpu"] = sweep.get("gpu")
+    out["cpu_threads"] = sweep.get("cpu_threads")
+    out["sweep"] = [{"nnz": r["nnz"], "cpu_seconds_per_iteration": r["cpu"]["seconds_per_iteration"],
+                     "gpu_seconds_per_iteration": r["gpu"]["seconds_per_iteration"],
+                     "gpu_effective_GBs": r["gpu"]["effective_GBs"]} for r in rows]
+    return out
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+
+    if len(sys.argv) != 3 or sys.argv[1] != "crossover":
+        raise SystemExit("usage: python router_fit.py crossover <calibrate output JSON>")
+    print(json.dumps(crossover_from_sweep(json.load(open(sys.argv[2]))), indent=1))

@@ -5,8 +5,9 @@ Router::decide (src/router/router.cpp:157-203) driven by the re-fitted model.
 test_router_sweep is a measurement run, skipped unless PDHG_ROUTER_SWEEP_OUT
 names the JSON file to write: every instance is solved by each of the
 reference's four LP engines (dual simplex, IPM and CPU PDHG on the host,
-pdhg-gpu on this engine) through the reference CLI built over this engine,
-with the same time limit for all; a run without a proof counts 4x the limit.
+pdhg-gpu on this engine, the PDHG engines followed by the reference's
+crossover) through the reference CLI built over this engine, with the same
+time limit for all; a run without a proof counts 4x the limit.
 Its output is committed as profiles/router_model_mi355x.json.
 
 test_committed_router_model_drives_the_reference_router loads that file into
@@ -47,18 +48,17 @@ def generated(tmp):
 
 
 def run(path, engine, tmp):
-    """One solve by `engine` under the time limit; the PDHG engines without the
-    crossover (the router adds that polish to their prediction itself,
-    router.cpp:181-183).  A run that outlives the limit by far is stopped and
-    counts as unproven."""
+    """One solve by `engine` under the time limit, with the reference's
+    defaults: a first-order answer counts once the crossover has turned it
+    into a certified vertex (without it solve() reports no proof).  A run
+    that outlives the limit by far is stopped and counts as unproven."""
     js = tmp / "r.json"
     if js.exists():
         js.unlink()
-    extra = ["--no-crossover"] if engine.startswith("pdhg") else []
     print(f"  {os.path.basename(str(path))} {engine}", flush=True)
     try:
         rc, out, err = I.cli(str(path), "--algo", engine, "--time", str(LIMIT), "--json", str(js), "--log", "0",
-                             *extra, timeout=LIMIT * 3 + 30)
+                             timeout=LIMIT * 3 + 30)
     except subprocess.TimeoutExpired:
         return None
     return json.loads(js.read_text()) if js.exists() else None
