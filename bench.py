@@ -225,12 +225,13 @@ def elapsed():
     return time.perf_counter() - T0
 
 
-def kernel_roofline(lp, costs, prec, row_map, tol):
+def kernel_roofline(lp, costs, prec, row_map, xcd_remap, tol):
     """Per-launch roofline of the iteration kernels at width 64: HIP events on
     the run's own stream over 200 launches each, after a short solve of the
     workload's first 64 cases with the tuned mapping."""
     _, kst, _ = P.solve_batch(lp, costs[:64], P.default_options(tolerance=tol, precision=prec, max_iterations=64 * 8,
-                                                                kernel_timing_reps=200, row_map=row_map))
+                                                                kernel_timing_reps=200, row_map=row_map,
+                                                                xcd_remap=xcd_remap))
     kern = {"k_col": (kst["col_kernel_bytes"], kst["col_kernel_ms"]),
             "k_row": (kst["row_kernel_bytes"], kst["row_kernel_ms"])}
     dom = max(kern, key=lambda k: kern[k][1])
@@ -246,7 +247,7 @@ def kernel_roofline(lp, costs, prec, row_map, tol):
         f"k_row {kern['k_row'][1] * 1e3:.1f} us ({kern['k_row'][0] / kern['k_row'][1] / 1e6:.0f} GB/s)")
     return {"bound": "hbm", "kernel": dom + ("_tiled" if tiled else ""), "achieved": achieved, "peak": PEAK_HBM_GBS,
             "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "algorithmic_bytes": dbytes,
-            "launch_us": dms * 1e3,
+            "launch_us": dms * 1e3, "xcd_remap": kst["col_xcd" if dom == "k_col" else "row_xcd"],
             "other_kernel": {k: {"launch_us": v[1] * 1e3, "algorithmic_bytes": v[0]} for k, v in kern.items()
                              if k != dom}}
 
@@ -277,6 +278,8 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64", "fp32auto"], default="fp64")
     ap.add_argument("--row-map", type=int, default=0, choices=[0, 1, 2, 3],
                     help="row mapping of the iteration kernels: 0 autotuned, 1 buckets, 2 one mapping, 3 LDS tiles")
+    ap.add_argument("--xcd-remap", type=int, default=-1, choices=[-1, 0, 1],
+                    help="workgroup placement: -1 autotuned, 0 round-robin over XCDs, 1 contiguous ranges per XCD")
     ap.add_argument("--cases", type=int, default=64, help="family cases solved per step (config C3: 64)")
     ap.add_argument("--cold", action="store_true", help="no learned warm start (batched cold start only)")
     ap.add_argument("--train", type=int, default=64, help="family parameters solved to fit the warm start")
@@ -338,9 +341,10 @@ def main():
     K = args.cases
     thetas, costs = P.gen_family(lp, SEED_THETA, K, 0.3)
     opts = P.default_options(tolerance=args.tol, precision=prec, max_iterations=args.max_iterations,
-                             progress_seconds=15.0, row_map=args.row_map, warm_fallback=args.warm_fallback)
+                             progress_seconds=15.0, row_map=args.row_map, xcd_remap=args.xcd_remap,
+                             warm_fallback=args.warm_fallback)
     plain_opts = P.default_options(tolerance=args.tol, precision=prec, max_iterations=args.max_iterations,
-                                   row_map=args.row_map)
+                                   row_map=args.row_map, xcd_remap=args.xcd_remap)
     log(f"workload: refinery m={lp.m} n={lp.n} nnz={lp.nnz}, {K} instances, precision {args.precision}")
     if args.plain_all:
         pl = plain_gpu(lp, costs, plain_opts, K)
@@ -358,7 +362,7 @@ def main():
         fit_s = time.perf_counter() - t
         log(f"warm-start fit: {fit_s:.1f} s ({args.train} parameters at tolerance {args.train_tol})")
 
-    roof = kernel_roofline(lp, costs, prec, args.row_map, args.tol)
+    roof = kernel_roofline(lp, costs, prec, args.row_map, args.xcd_remap, args.tol)
     spmv = spmv_rates(lp)
     log("spmv (op-level k_spmv, scaled refinery matrix): " + ", ".join(
         f"{k} A x {v['Ax']['GBs']:.0f} / A^T y {v['ATy']['GBs']:.0f} GB/s" for k, v in spmv.items()))
