@@ -131,12 +131,18 @@ uint64_t structure_hash(const Prepared& P) {
 }
 
 // Row mappings the autotuner times.  At B = 64 a wavefront serves one row
-// (lane = instance), so only the XCD remap is open; below, the lanes per row
-// (nnz-per-lane target, which sets the row tile S*B) and bucketing are too,
-// and at B = 16 and 32 the grouped form (one row per B-lane group).
+// (lane = instance); below, the lanes per row (nnz-per-lane target, which
+// sets the row tile S*B) and bucketing are open, and at B = 16 and 32 the
+// grouped form (one row per B-lane group).  From B = 16 on the XCD remap is
+// always on: without it every XCD's L2 fetches the gathered columns of rows
+// spread over the whole matrix (on the refinery matrix at B = 64, 2.8x the
+// distinct gathered bytes for the LDS tiles and 6x for one row per
+// wavefront, by FETCH_SIZE), a cost that timings inside the Infinity Cache
+// do not show but HBM-resident runs pay.
 std::vector<MapSetup> map_candidates(int B) {
   std::vector<MapSetup> c;
   for (bool xcd : {false, true}) {
+    if (!xcd && B >= 16) continue;
     if (B == 64) {
       for (int rpw : {1, 2}) c.push_back(MapSetup{4, 1, xcd, false, rpw});
       c.push_back(MapSetup{4, 1, xcd, false, 1, true});  // rows in LDS-staged tiles

@@ -1748,9 +1748,11 @@ class OpBackend final : public OpBackendBase {
   }
   // Times every candidate mapping of one SpMV (8 launches after a warm one,
   // HIP events on the backend's stream, zero vectors) and keeps the fastest.
+  // From B = 16 on the XCD remap stays on, as in the engine's tuner.
   PlanHost tune_spmv(const Csr& M, const int* ptr, const int* idx, const T* val, const T* in, T* out) {
     std::vector<MapSetup> cand;
     for (bool xcd : {false, true}) {
+      if (!xcd && B_ >= 16) continue;
       if (B_ == 64) {
         for (int rpw : {1, 2}) cand.push_back(MapSetup{4, 1, xcd, false, rpw});
         continue;
