@@ -374,6 +374,7 @@ def main():
     if os.path.exists(pf) and prec == P.FP64 and K == 64:
         d = json.load(open(pf))
         plain_full = {k: d[k] for k in ("solves_per_s", "cases", "certified", "seconds")}
+        plain_full["source"] = "profiles/plain_gpu_c3_fp64.json, recorded by bench.py --plain-all on this workload"
 
     # Warmup and timed steps inside the wall budget.
     step_s = []
