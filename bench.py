@@ -5,31 +5,37 @@ Workload (one "step"): the 64-case price family of the refinery-style LP
 nonzeros, rows/columns scaled by 10^U(-3,3); costs c_k = c0 + theta_k d on
 four crude purchase prices, theta_k ~ U[-0.3, 0.3]^4, seed 4) solved by ONE
 batched run of width 64 (FP64 iterates, hipGraphs of 64 iterations,
-autotuned row mapping) to relative KKT <= 1e-6 from the learned warm start,
-with the per-case cold-start fallback (a warm case not converged after 256k
-iterations restarts in its lane from the reference's cold start), then every
-answer certified (implied bounds + rigorous safe dual bound, acceptance
-|obj - LB| <= 1e-4 (1 + |obj|)).  A step is: MLP inference theta -> POD
-coefficients (BF16 MFMA), BF16-MFMA decode of the starting points, upload,
+autotuned row mapping) to relative KKT <= 1e-6, then every answer certified
+(implied bounds + rigorous safe dual bound, acceptance
+|obj - LB| <= 1e-4 (1 + |obj|)).  A step is: (warm start only) MLP inference
+theta -> POD coefficients and BF16-MFMA decode of the starting points, upload,
 batched solve, download, certification.  The value is certified solves per
 second of wall time over the timed steps (host <-> device transfers
 included).
+
+Start of a step: the learned warm start (with its per-case cold-start
+fallback: a warm case not converged after 256k iterations restarts in its
+lane from the reference's cold start) or the reference's cold start.  The
+warmup steps run warm, then one run of the same family runs cold; the timed
+steps use whichever certified more solves per second in this run (autotuning
+seconds excluded), and the line reports both runs and the choice
+(--start forces one).
 
 Before the timer: the warm-start model is fitted once on 64 other parameters
 of the family (solved at the same tolerance from a cold start; POD basis, MLP
 trained on the GPU); its cost is reported as warm_start_fit_seconds.  Then
 the per-launch roofline of the dominant iteration kernel, the standalone SpMV
-rates, the reference's 16-thread CPU PDHG on a bounded sample of the same
-workload, and plain single-instance GPU PDHG on a sample of the cases.
+rates on the refinery matrix and on a refinery LP beyond the 256 MB Infinity
+Cache, plain single-instance GPU PDHG on a sample of the cases, and the
+reference's 16-thread CPU PDHG on a bounded sample of the same workload.
 
-Wall budget: the whole run stays inside --budget seconds (default 540, below
-the 600 s a driver gives it): warmup and timed steps stop early when the
-next step would not fit, and the line reports the steps actually timed.
-Before the timed steps, after one warmup: the reference's CPU PDHG and the
-same family from a cold start (batched_cold).  After them, when the budget
-still allows: config C4's 256 uneven LPs in one packed run.
+Wall budget: the whole run stays inside --budget seconds (default 330, well
+below the 600 s a driver gives it): warmup and timed steps stop early when
+the next step would not fit, and the line reports the steps actually timed.
+After them, when the budget still allows: config C4's 256 uneven LPs in one
+packed run.
 
-    python bench.py [--steps 20] [--warmup 5] [--gpus 1] [--budget 540]
+    python bench.py [--steps 20] [--warmup 5] [--gpus 1] [--budget 330]
 """
 import argparse
 import json
@@ -177,6 +183,37 @@ def cpu_baseline(lp, costs, gpu_iters, seconds, threads):
     }
 
 
+def spmv_beyond_cache(periods, reps=50):
+    """Standalone CSR SpMV (the op-level backend's autotuned k_spmv, width 1)
+    on a refinery LP of `periods` periods whose matrix exceeds the 256 MB
+    Infinity Cache (1200 periods: 55.6M nonzeros, 445 MB of FP32 values and
+    indices), A x and A^T y in FP32 and FP64, HIP events over `reps` launches.
+    The matrix is the generator's unscaled one (the kernel's rate does not
+    depend on the values), which saves the host set-up of the scaling."""
+    import scipy.sparse as sp
+
+    lp = P.gen_refinery(SEED_LP, periods=periods)
+    A = sp.csc_matrix((lp.value, lp.index, lp.start), shape=(lp.m, lp.n)).tocsr()
+    A.sort_indices()
+    AT = A.T.tocsr()
+    AT.sort_indices()
+    rng = np.random.default_rng(0)
+    out = {"workload": f"refinery LP, {periods} periods: m={lp.m} n={lp.n} nnz={lp.nnz}, width 1"}
+    for prec, name, vs in ((P.FP32, "fp32", 4), (P.FP64, "fp64", 8)):
+        be = P.Backend(A, AT, np.ones(lp.m), np.ones(lp.n), k=1, precision=prec)
+        be.set(P.V_X, rng.standard_normal(lp.n))
+        be.set(P.V_Y, rng.standard_normal(lp.m))
+        for transpose, key, rows, cols in ((False, "Ax", lp.m, lp.n), (True, "ATy", lp.n, lp.m)):
+            ms = be.time_spmv(transpose, reps)
+            nbytes = lp.nnz * (vs + 4) + (rows + 1) * 4 + (rows + cols) * vs
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            out[f"{name}_{key}"] = {"us": ms * 1e3, "GBs": gbs, "bytes": nbytes, "frac": gbs / PEAK_HBM_GBS}
+        be.close()
+    log("spmv beyond the Infinity Cache: " + ", ".join(
+        f"{k} {v['GBs']:.0f} GB/s ({v['frac']:.2f})" for k, v in out.items() if k != "workload"))
+    return out
+
+
 def large_lp_rates(periods, reps=50):
     """Kernels on a refinery LP too large for the 256 MB Infinity Cache
     (`periods` periods of the staircase: 1200 gives 55.6M nonzeros, 445 MB of
@@ -273,7 +310,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--budget", type=float, default=540.0,
+    ap.add_argument("--budget", type=float, default=330.0,
                     help="wall seconds for the whole run; warmup and timed steps stop when the next would not fit")
     ap.add_argument("--precision", choices=["fp32", "fp64", "fp32auto"], default="fp64")
     ap.add_argument("--row-map", type=int, default=0, choices=[0, 1, 2, 3],
@@ -290,7 +327,12 @@ def main():
     ap.add_argument("--no-cold-batch", action="store_true", help="skip the cold-start batched comparison run")
     ap.add_argument("--no-packed", action="store_true", help="skip the packed run of 256 uneven LPs")
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--plain-sample", type=int, default=2, help="instances solved one by one for the plain baseline")
+    ap.add_argument("--plain-sample", type=int, default=1, help="instances solved one by one for the plain baseline")
+    ap.add_argument("--hbm-periods", type=int, default=1200,
+                    help="periods of the refinery LP for the SpMV rates beyond the Infinity Cache (0: skip)")
+    ap.add_argument("--start", choices=["measured", "warm", "cold"], default="measured",
+                    help="start of the timed steps: the faster of the warmup (learned warm start) and the cold "
+                         "batch as measured in this run, or forced")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
@@ -376,11 +418,20 @@ def main():
         plain_full = {k: d[k] for k in ("solves_per_s", "cases", "certified", "seconds")}
         plain_full["source"] = "profiles/plain_gpu_c3_fp64.json, recorded by bench.py --plain-all on this workload"
 
-    # Warmup and timed steps inside the wall budget.
+    hbm = None
+    if args.hbm_periods > 0:
+        hbm = spmv_beyond_cache(args.hbm_periods)
+
+    # Warmup and timed steps inside the wall budget.  The warmup steps start
+    # from the learned warm start; after them one run of the same family from
+    # the cold start.  The timed steps then use whichever of the two certified
+    # more solves per second of its run here (autotuning seconds excluded), or
+    # the start forced by --start.
     step_s = []
     warm_done = 0
     reserve = 10.0  # the CPU baseline's and the report's own seconds come after the steps
     cpu_est = 0.0 if args.no_cpu else 2.2 * args.cpu_seconds
+    warm_runs = []
     for w in range(args.warmup):
         if warm_done >= 1 and elapsed() + (1 + args.steps) * max(step_s) + cpu_est + reserve > args.budget:
             break  # one warmup at least; more only when every requested timed step still fits
@@ -388,8 +439,11 @@ def main():
         res, st, cert, ok = family_step(lp, thetas, costs, opts, ws)
         step_s.append(time.perf_counter() - t)
         warm_done += 1
-        log(f"warmup {w}: {step_s[-1]:.2f} s, {st['iterations']} iterations, {ok.sum()}/{K} certified, "
-            f"{st['warm_fallbacks']} cold fallbacks")
+        warm_runs.append({"seconds": step_s[-1], "tune_seconds": st["tune_seconds"], "certified": int(ok.sum()),
+                          "iterations": st["iterations"], "cold_fallbacks": st["warm_fallbacks"],
+                          "median_instance_iterations": float(np.median([r.iterations for r in res]))})
+        log(f"warmup {w}: {step_s[-1]:.2f} s ({st['tune_seconds']:.2f} s autotuning), {st['iterations']} iterations, "
+            f"{ok.sum()}/{K} certified, {st['warm_fallbacks']} cold fallbacks")
 
     # The reference's CPU PDHG on the same family, extrapolated to the
     # iterations of the slowest case of a step (from the warmup when there is one).
@@ -405,17 +459,33 @@ def main():
         cpu = run_cpu(int(min(max(r.iterations for r in res), args.max_iterations)))
         cpu_est = 0.0
 
-    # The same batched solve from a cold start (one run, for the warm/cold
-    # ratio), before the timed steps so that the budget cannot drop it.
+    # The same batched solve from a cold start (one run), before the timed
+    # steps so that the budget cannot drop it.
     cold = None
     if ws is not None and not args.no_cold_batch:
         t = time.perf_counter()
         res_c, st_c, _, ok_c = family_step(lp, thetas, costs, opts, None)
         el = time.perf_counter() - t
-        cold = {"solves_per_s": int(ok_c.sum()) / el, "seconds": el, "iterations": st_c["iterations"],
-                "certified": int(ok_c.sum()),
+        cold = {"solves_per_s": int(ok_c.sum()) / el, "seconds": el, "tune_seconds": st_c["tune_seconds"],
+                "iterations": st_c["iterations"], "certified": int(ok_c.sum()),
                 "median_instance_iterations": float(np.median([r.iterations for r in res_c]))}
         log(f"batched cold start: {el:.2f} s, {st_c['iterations']} iterations, {ok_c.sum()}/{K} certified")
+
+    def rate(r):
+        return r["certified"] / max(r["seconds"] - r["tune_seconds"], 1e-9)
+
+    step_ws, choice = ws, None
+    if ws is not None and cold is not None and warm_runs:
+        warm_rate = max(rate(r) for r in warm_runs)
+        cold_rate = rate(cold)
+        use_warm = args.start == "warm" or (args.start == "measured" and warm_rate >= cold_rate)
+        step_ws = ws if use_warm else None
+        choice = {"rule": args.start, "warm_certified_per_s": warm_rate, "cold_certified_per_s": cold_rate,
+                  "timed_start": "learned warm start" if use_warm else "cold"}
+        log(f"start of the timed steps: {choice['timed_start']} (warm {warm_rate:.3f}, cold {cold_rate:.3f} "
+            f"certified solves/s, rule {args.start})")
+        if not use_warm:
+            step_s = [cold["seconds"]]  # the budget's step estimate follows the chosen start
 
     certified, iters, kkt, gaps, inst_iters, fallbacks = 0, [], [], [], [], 0
     up_bytes = down_bytes = 0
@@ -425,7 +495,7 @@ def main():
         if s >= 1 and elapsed() + est + cpu_est + reserve > args.budget:
             break
         ts = time.perf_counter()
-        res, st, cert, ok = family_step(lp, thetas, costs, opts, ws)
+        res, st, cert, ok = family_step(lp, thetas, costs, opts, step_ws)
         timed.append(time.perf_counter() - ts)
         step_s.append(timed[-1])
         up_bytes += st["bytes_uploaded"]
@@ -465,7 +535,9 @@ def main():
                   "max_instance_iterations": int(max(r.iterations for r in res_p))}
         log(f"packed uneven batch: {conv_p}/{len(lps)} at rel-KKT <= {args.tol:g}, {cert_p} certified, in {el:.2f} s")
 
-    mode = "batched+warm" if ws is not None else "batched"
+    mode = "batched+warm" if step_ws is not None else "batched"
+    if ws is not None and step_ws is None:
+        mode = "batched, cold start measured faster than the learned warm start"
     tol_label = ("%g" % args.tol).replace("e-0", "e-")  # 1e-06 -> 1e-6
     plain_rate = plain_full["solves_per_s"] if plain_full else plain["solves_per_s"]
     out = {
@@ -483,8 +555,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"config C3: {K}-case price family of the refinery LP (m={lp.m} n={lp.n} "
                                f"nnz={lp.nnz}) in one batched run of width 64, tol {args.tol}",
-                   "cases": K, "batch_width": 64, "start": "learned warm start" if ws is not None else "cold",
-                   "warm_fallback_iterations": args.warm_fallback if ws is not None else None,
+                   "cases": K, "batch_width": 64, "start": "learned warm start" if step_ws is not None else "cold",
+                   "warm_fallback_iterations": args.warm_fallback if step_ws is not None else None,
                    "warm_start_training": (f"{args.train} family parameters at rel-KKT <= {args.train_tol:g}"
                                            if ws is not None else None),
                    "iterations": gpu_iters, "median_instance_iterations": float(np.median(inst_iters)),
@@ -496,12 +568,15 @@ def main():
         "roofline": roof,
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "spmv": spmv,
+        "spmv_beyond_infinity_cache": hbm,
         "bytes_per_step": {"uploaded": up_bytes / steps, "downloaded": down_bytes / steps},
         "plain_gpu_solves_per_s": plain_rate,
         "plain_gpu_sample": plain,
         "plain_gpu_all_cases": plain_full,
         "vs_plain_gpu": (value / plain_rate) if plain_rate else None,
+        "batched_warm": warm_runs[-1] if warm_runs and ws is not None else None,
         "batched_cold": cold,
+        "start_choice": choice,
         "packed_uneven": packed,
         "warm_start_fit_seconds": fit_s if ws is not None else None,
         "max_rel_kkt": max(kkt),
