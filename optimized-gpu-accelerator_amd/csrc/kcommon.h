@@ -44,11 +44,10 @@ struct RowPlan {
   int xcd;                         // 1: contiguous row ranges per XCD
   int grp;                         // 1: one row per B-lane group, entries via the scalar cache
   int rpw;                         // B = 64: rows per wavefront (1 or 2)
-  // B = 64 tiles (one workgroup each): rows perm[tileRow[t] .. tileRow[t+1]),
-  // whose distinct columns ucols[tileCol[t] .. tileCol[t+1]) are staged in LDS.
-  const int* tileRow;
-  const int* tileCol;
-  const int* ucols;
+  // B = 64 tiles (one workgroup each): one record per (tile, wavefront)
+  // with the wavefront's row, its entry range and the tile columns it
+  // stages in LDS (tile_descriptors in tiles.h).
+  const int* desc;
 };
 
 // Workgroup ids are dispatched round-robin over the 8 XCDs; this bijection
@@ -231,33 +230,53 @@ __device__ inline Acc row_dot_lds(const int* __restrict__ loc, const V* __restri
   return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
-// Staging of tile t of a B = 64 plan in LDS: column u - u0 of lane b holds
-// src row ucols[u] of lane b (src may differ per lane).  tile_load issues a
-// wavefront's share of the loads, tile_store writes them to LDS; a kernel
-// issues its other loads in between, then meets the others at a barrier.
+// Staging of tile t of a B = 64 plan in LDS: tile position u of lane b
+// holds the gathered row of the tile's u-th column for lane b (src may
+// differ per lane).  Wavefront w stages positions w, w + kTileWaves, ...
+// Its record (tile_descriptors) comes in one round of scalar loads with no
+// load before it, so the gathers start one memory latency after launch.
 template <typename T>
 constexpr int kStageCols = (kTileCols<T> + kTileWaves - 1) / kTileWaves;  // per wavefront at most
+template <typename T>
+constexpr int kDescInts = desc_stride(kStageCols<T>);
 
-template <bool SPLIT, typename T>
-__device__ inline void tile_load(const RowPlan& p, int t, const T* src, int b, T (&v)[kStageCols<T>]) {
-  const int u0 = p.tileCol[t], u1 = p.tileCol[t + 1];
+// The wavefront's record of tile t, read into registers (wave-uniform).
+template <typename T>
+struct TileRec {
+  int row, e0, e1, ns;
+  int col[kStageCols<T>];
+};
+
+template <typename T>
+__device__ inline TileRec<T> tile_record(const RowPlan& p, int t) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int* __restrict__ d = p.desc + (static_cast<size_t>(t) * kTileWaves + wave) * kDescInts<T>;
+  TileRec<T> r;
+  r.row = d[0];
+  r.e0 = d[1];
+  r.e1 = d[2];
+  r.ns = d[3];
 #pragma unroll
-  for (int k = 0; k < kStageCols<T>; ++k) {
-    const int uu = u0 + wave + k * kTileWaves;
-    if (uu < u1) v[k] = vld<SPLIT, 6>(src, p.ucols[uu], b);
-  }
+  for (int k = 0; k < kStageCols<T>; ++k) r.col[k] = d[4 + k];
+  return r;
+}
+
+// tile_load issues a wavefront's share of the staging loads, tile_store
+// writes them to LDS; a kernel issues its other loads in between, then meets
+// the others at the barrier.
+template <bool SPLIT, typename T>
+__device__ inline void tile_load(const TileRec<T>& r, const T* src, int b, T (&v)[kStageCols<T>]) {
+#pragma unroll
+  for (int k = 0; k < kStageCols<T>; ++k)
+    if (k < r.ns) v[k] = vld<SPLIT, 6>(src, r.col[k], b);
 }
 
 template <typename T>
-__device__ inline void tile_store(const RowPlan& p, int t, const T (&v)[kStageCols<T>], T* tile, int b) {
-  const int u0 = p.tileCol[t], u1 = p.tileCol[t + 1];
+__device__ inline void tile_store(const TileRec<T>& r, const T (&v)[kStageCols<T>], T* tile, int b) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
 #pragma unroll
-  for (int k = 0; k < kStageCols<T>; ++k) {
-    const int uu = u0 + wave + k * kTileWaves;
-    if (uu < u1) tile[(uu - u0) * 64 + b] = v[k];
-  }
+  for (int k = 0; k < kStageCols<T>; ++k)
+    if (k < r.ns) tile[(wave + k * kTileWaves) * 64 + b] = v[k];
   __syncthreads();
 }
 

@@ -248,90 +248,65 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
 }
 
 // B = 64 tiled forms of k_col / k_row (no check quantities): a workgroup
-// takes one tile (kTileRows rows in locality order), each wavefront every
-// kTileWaves-th row of it.  One round of loads brings the tile's distinct
-// gathered rows (staged in LDS) and the wavefront's update operands; after
-// one barrier the dot products read LDS.  idx holds tile-local columns.
+// takes one tile (up to kTileWaves rows in locality order), each wavefront
+// one row of it.  The wavefront's record (row, entry range, staged columns)
+// arrives in one round of scalar loads; one round of vector loads then
+// brings the tile's distinct gathered rows (staged in LDS) and the row's
+// update operands; after one barrier the dot products read LDS.  loc holds
+// tile-local columns.
 template <typename T>
-__global__ __launch_bounds__(kTileWG) void k_col_tiled(RowPlan plan, const int* __restrict__ ptr,
-                                                   const int* __restrict__ loc, const T* __restrict__ val,
-                                                   const T* __restrict__ y, const T* __restrict__ yp,
-                                                   T* __restrict__ x, T* __restrict__ xp, T* __restrict__ x0,
-                                                   const T* __restrict__ c, Ld lc, const T* __restrict__ lo,
-                                                   const T* __restrict__ up, Ld lb, Inst st, int tloc, double rho,
-                                                   double* __restrict__, int) {
+__global__ __launch_bounds__(kTileWG) void k_col_tiled(RowPlan plan, const int* __restrict__ loc,
+                                                   const T* __restrict__ val, const T* __restrict__ y,
+                                                   const T* __restrict__ yp, T* __restrict__ x, T* __restrict__ xp,
+                                                   T* __restrict__ x0, const T* __restrict__ c, Ld lc,
+                                                   const T* __restrict__ lo, const T* __restrict__ up, Ld lb, Inst st,
+                                                   int tloc, double rho) {
   constexpr bool SPL = kSplit<T, 6>;
-  constexpr int RW = (kTileRows + kTileWaves - 1) / kTileWaves;
   __shared__ T tile[kTileCols<T> * 64];
   const int t = block_id(plan);
   const int b = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const TileRec<T> r = tile_record<T>(plan, t);
   const long long it = st.iter[b] + tloc;
   const bool rst = st.applyAt[b] == it;
   T stage[kStageCols<T>];
-  tile_load<SPL>(plan, t, rst ? yp : y, b, stage);
-  const int r0 = plan.tileRow[t] + wave, r1 = plan.tileRow[t + 1];
-  int rows[RW];
-  ColOps<T> ops[RW];
-#pragma unroll
-  for (int k = 0; k < RW; ++k) {
-    const int r = r0 + k * kTileWaves;
-    rows[k] = r < r1 ? plan.perm[r] : -1;
-    if (rows[k] >= 0) ops[k] = col_load<T, 6>(rows[k], b, rst, x, xp, x0, c, lc, lo, up, lb);
-  }
-  tile_store(plan, t, stage, tile, b);
-  const bool live = !st.done[b];
+  tile_load<SPL>(r, rst ? yp : y, b, stage);
+  const int j = r.row;
+  ColOps<T> ops;
+  if (j >= 0) ops = col_load<T, 6>(j, b, rst, x, xp, x0, c, lc, lo, up, lb);
+  tile_store(r, stage, tile, b);
+  if (j < 0) return;
+  const T acc = row_dot_lds<T>(loc, val, tile, r.e0, r.e1, b);
   double q[2] = {0.0, 0.0};
-#pragma unroll
-  for (int k = 0; k < RW; ++k) {
-    if (rows[k] < 0) continue;
-    const int j = rows[k];
-    const T acc = row_dot_lds<T>(loc, val, tile, ptr[j], ptr[j + 1], b);
-    if (live) col_apply<T, 6, false>(j, acc, ops[k], b, rst, it, x, xp, x0, st, rho, q);
-  }
+  if (!st.done[b]) col_apply<T, 6, false>(j, acc, ops, b, rst, it, x, xp, x0, st, rho, q);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kTileWG) void k_row_tiled(RowPlan plan, const int* __restrict__ ptr,
-                                                   const int* __restrict__ loc, const T* __restrict__ val,
-                                                   const double* __restrict__, const T* __restrict__ xp,
+__global__ __launch_bounds__(kTileWG) void k_row_tiled(RowPlan plan, const int* __restrict__ loc,
+                                                   const T* __restrict__ val, const T* __restrict__ xp,
                                                    T* __restrict__ y, T* __restrict__ yp, T* __restrict__ y0,
                                                    T* __restrict__ ax, T* __restrict__ axp, T* __restrict__ ax0,
                                                    const T* __restrict__ rl, const T* __restrict__ ru,
                                                    const double* __restrict__ rl64, const double* __restrict__ ru64,
                                                    Ld lr, const double* __restrict__ rfac, Inst st, int tloc,
-                                                   double rho, double* __restrict__, int) {
+                                                   double rho) {
   constexpr bool SPL = kSplit<T, 6>;
-  constexpr int RW = (kTileRows + kTileWaves - 1) / kTileWaves;
   __shared__ T tile[kTileCols<T> * 64];
   const int t = block_id(plan);
   const int b = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const TileRec<T> r = tile_record<T>(plan, t);
   const long long it = st.iter[b] + tloc;
   const bool rst = st.applyAt[b] == it;
   T stage[kStageCols<T>];
-  tile_load<SPL>(plan, t, xp, b, stage);
-  const int r0 = plan.tileRow[t] + wave, r1 = plan.tileRow[t + 1];
-  int rows[RW];
-  RowOps<T> ops[RW];
-#pragma unroll
-  for (int k = 0; k < RW; ++k) {
-    const int r = r0 + k * kTileWaves;
-    rows[k] = r < r1 ? plan.perm[r] : -1;
-    if (rows[k] >= 0) ops[k] = row_load<T, 6>(rows[k], b, rst, y, yp, y0, ax, axp, ax0, rl, ru, lr);
-  }
-  tile_store(plan, t, stage, tile, b);
-  const bool live = !st.done[b];
+  tile_load<SPL>(r, xp, b, stage);
+  const int i = r.row;
+  RowOps<T> ops;
+  if (i >= 0) ops = row_load<T, 6>(i, b, rst, y, yp, y0, ax, axp, ax0, rl, ru, lr);
+  tile_store(r, stage, tile, b);
+  if (i < 0) return;
+  const T acc = row_dot_lds<T>(loc, val, tile, r.e0, r.e1, b);
   double q[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int k = 0; k < RW; ++k) {
-    if (rows[k] < 0) continue;
-    const int i = rows[k];
-    const T acc = row_dot_lds<T>(loc, val, tile, ptr[i], ptr[i + 1], b);
-    if (live)
-      row_apply<T, 6, false>(i, acc, 0.0, ops[k], b, rst, it, y, yp, y0, ax, axp, ax0, rl64, ru64, lr, rfac, st,
-                             rho, q);
-  }
+  if (!st.done[b])
+    row_apply<T, 6, false>(i, acc, 0.0, ops, b, rst, it, y, yp, y0, ax, axp, ax0, rl64, ru64, lr, rfac, st, rho, q);
 }
 
 // Column KKT terms at (x+, y+) in FP64 (reference pdhg.cpp:181-196):
@@ -668,6 +643,7 @@ struct PlanHost {
   bool tiled = false;
   RowPlan tplan{};
   TileSet tiles;
+  std::vector<int> desc;  // tile_descriptors, kDescInts ints per (tile, wavefront)
   int* dLoc = nullptr;
 };
 
@@ -750,6 +726,7 @@ PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms, int tileCols = 0
     h.tiled = true;
     h.tplan = h.plan;
     h.tplan.xcd = ms.xcd ? 1 : 0;
+    h.desc = tile_descriptors(M, h.tiles, (tileCols + kTileWaves - 1) / kTileWaves);
   }
   return h;
 }
@@ -825,8 +802,12 @@ struct Kern {
   ColCheckFn colcheck[2];
   SpmvFn spmv[2];  // the run's vector layout
   SpmvFn spmvStd;  // element (i, b) at i*B + b (op-level backend)
-  ColFn colTiled = nullptr;  // B = 64 tiles
-  RowFn rowTiled = nullptr;
+  using ColTiledFn = void (*)(RowPlan, const int*, const T*, const T*, const T*, T*, T*, T*, const T*, Ld, const T*,
+                              const T*, Ld, Inst, int, double);
+  using RowTiledFn = void (*)(RowPlan, const int*, const T*, const T*, T*, T*, T*, T*, T*, T*, const T*, const T*,
+                              const double*, const double*, Ld, const double*, Inst, int, double);
+  ColTiledFn colTiled = nullptr;  // B = 64 tiles
+  RowTiledFn rowTiled = nullptr;
 };
 
 template <typename T, int LOGB>
@@ -1356,10 +1337,8 @@ class RunImpl final : public DeviceRun {
     if (!h.perm.empty()) h.dPerm = upload_int(h.perm);
     h.plan.perm = h.dPerm;
     if (h.tiled) {
-      h.tplan.perm = upload_int(h.tiles.perm);
-      h.tplan.tileRow = upload_int(h.tiles.tileRow);
-      h.tplan.tileCol = upload_int(h.tiles.tileCol);
-      h.tplan.ucols = upload_int(h.tiles.ucols.empty() ? std::vector<int>{0} : h.tiles.ucols);
+      h.tplan.perm = nullptr;
+      h.tplan.desc = upload_int(h.desc);
       h.dLoc = upload_int(h.tiles.loc);
     }
   }
@@ -1421,9 +1400,8 @@ class RunImpl final : public DeviceRun {
   }
   void launch_col(bool check, int tloc) {
     if (!check && colPlan_.tiled) {
-      hipLaunchKernelGGL(k_.colTiled, dim3(colPlan_.tiles.tiles), dim3(kTileWG), 0, stream_, colPlan_.tplan, tPtr_,
-                         colPlan_.dLoc, tVal_, y_, yp_, x_, xp_, x0_, c_, lc_, lo_, up_, lb_, st_, tloc, s_.rho,
-                         colPart_, 0);
+      hipLaunchKernelGGL(k_.colTiled, dim3(colPlan_.tiles.tiles), dim3(kTileWG), 0, stream_, colPlan_.tplan,
+                         colPlan_.dLoc, tVal_, y_, yp_, x_, xp_, x0_, c_, lc_, lo_, up_, lb_, st_, tloc, s_.rho);
       return;
     }
     const int cr = colPlan_.plan.rpw - 1;
@@ -1433,9 +1411,9 @@ class RunImpl final : public DeviceRun {
   }
   void launch_row(bool check, int tloc) {
     if (!check && rowPlan_.tiled) {
-      hipLaunchKernelGGL(k_.rowTiled, dim3(rowPlan_.tiles.tiles), dim3(kTileWG), 0, stream_, rowPlan_.tplan, aPtr_,
-                         rowPlan_.dLoc, aVal_, aVal64_, xp_, y_, yp_, y0_, ax_, axp_, ax0_, rl_, ru_, rl64_, ru64_,
-                         lr_, rfac_, st_, tloc, s_.rho, rowPart_, 0);
+      hipLaunchKernelGGL(k_.rowTiled, dim3(rowPlan_.tiles.tiles), dim3(kTileWG), 0, stream_, rowPlan_.tplan,
+                         rowPlan_.dLoc, aVal_, xp_, y_, yp_, y0_, ax_, axp_, ax0_, rl_, ru_, rl64_, ru64_, lr_, rfac_,
+                         st_, tloc, s_.rho);
       return;
     }
     const int rr = rowPlan_.plan.rpw - 1;

@@ -96,4 +96,33 @@ inline bool build_tiles(const Csr& M, int maxCols, TileSet& t) {
   return t.tiles > 0;
 }
 
+// One record per (tile, wavefront) for the tiled kernels, so that a
+// wavefront gets everything it needs before its gathers in one round of
+// scalar loads: [0] its row (-1: none), [1], [2] the row's entry range,
+// [3] the number of tile columns it stages, [4 + k] the k-th of them (tile
+// position wave + k * kTileWaves, the matrix column ucols[tileCol[t] +
+// position]).  Records are `desc_stride(stageCols)` ints apart, a multiple of
+// eight so that each starts on a 32-byte boundary.
+constexpr int desc_stride(int stageCols) { return (4 + stageCols + 7) / 8 * 8; }
+
+inline std::vector<int> tile_descriptors(const Csr& M, const TileSet& t, int stageCols) {
+  static_assert(kTileRows <= kTileWaves, "one row per wavefront");
+  const int stride = desc_stride(stageCols);
+  std::vector<int> d(static_cast<size_t>(t.tiles) * kTileWaves * stride, 0);
+  for (int tt = 0; tt < t.tiles; ++tt) {
+    const int r0 = t.tileRow[tt], r1 = t.tileRow[tt + 1], u0 = t.tileCol[tt], u1 = t.tileCol[tt + 1];
+    for (int w = 0; w < kTileWaves; ++w) {
+      int* o = &d[(static_cast<size_t>(tt) * kTileWaves + w) * stride];
+      const int j = r0 + w < r1 ? t.perm[r0 + w] : -1;
+      o[0] = j;
+      o[1] = j >= 0 ? M.ptr[j] : 0;
+      o[2] = j >= 0 ? M.ptr[j + 1] : 0;
+      int ns = 0;
+      for (int k = 0; k < stageCols && u0 + w + k * kTileWaves < u1; ++k) o[4 + ns++] = t.ucols[u0 + w + k * kTileWaves];
+      o[3] = ns;
+    }
+  }
+  return d;
+}
+
 }  // namespace pdhgamd

@@ -1,7 +1,9 @@
 // Invariants of the width-64 row tiles (tiles.h), checked on the host:
 // every row in exactly one tile, at most kTileRows rows and maxCols distinct
 // columns per tile, loc[e] naming entry e's column among its tile's columns,
-// and the refusal of a matrix with a row longer than maxCols.
+// the refusal of a matrix with a row longer than maxCols, and the
+// per-(tile, wavefront) records: each wavefront's row and entry range, and
+// every tile column staged by exactly one wavefront at its tile position.
 // Usage: tile_check [matrix.bin ...]   (int32 rows, cols, nnz, ptr, idx)
 #include <cstdio>
 #include <cstdlib>
@@ -66,6 +68,38 @@ static bool check(const Csr& M, int maxCols, const char* name) {
       std::printf("%s: row %d in no tile\n", name, i);
       return false;
     }
+  const int stageCols = (maxCols + pdhgamd::kTileWaves - 1) / pdhgamd::kTileWaves;
+  const int stride = pdhgamd::desc_stride(stageCols);
+  const std::vector<int> d = pdhgamd::tile_descriptors(M, t, stageCols);
+  if (d.size() != static_cast<size_t>(t.tiles) * pdhgamd::kTileWaves * stride || stride % 8 != 0) {
+    std::printf("%s: descriptor size\n", name);
+    return false;
+  }
+  for (int k = 0; k < t.tiles; ++k) {
+    const int r0 = t.tileRow[k], r1 = t.tileRow[k + 1], u0 = t.tileCol[k], u1 = t.tileCol[k + 1];
+    std::vector<int> staged(u1 - u0, 0);
+    for (int w = 0; w < pdhgamd::kTileWaves; ++w) {
+      const int* o = &d[(static_cast<size_t>(k) * pdhgamd::kTileWaves + w) * stride];
+      const int j = r0 + w < r1 ? t.perm[r0 + w] : -1;
+      if (o[0] != j || (j >= 0 && (o[1] != M.ptr[j] || o[2] != M.ptr[j + 1])) || o[3] < 0 || o[3] > stageCols) {
+        std::printf("%s: tile %d wave %d record\n", name, k, w);
+        return false;
+      }
+      for (int q = 0; q < o[3]; ++q) {
+        const int pos = w + q * pdhgamd::kTileWaves;
+        if (pos >= u1 - u0 || o[4 + q] != t.ucols[u0 + pos]) {
+          std::printf("%s: tile %d wave %d stages a wrong column\n", name, k, w);
+          return false;
+        }
+        staged[pos]++;
+      }
+    }
+    for (int pos = 0; pos < u1 - u0; ++pos)
+      if (staged[pos] != 1) {
+        std::printf("%s: tile %d column %d staged %d times\n", name, k, pos, staged[pos]);
+        return false;
+      }
+  }
   return true;
 }
 
