@@ -262,13 +262,13 @@ def elapsed():
     return time.perf_counter() - T0
 
 
-def kernel_roofline(lp, costs, prec, row_map, xcd_remap, tol):
+def kernel_roofline(lp, costs, prec, row_map, xcd_remap, tol, graph=1):
     """Per-launch roofline of the iteration kernels at width 64: HIP events on
     the run's own stream over 200 launches each, after a short solve of the
     workload's first 64 cases with the tuned mapping."""
     _, kst, _ = P.solve_batch(lp, costs[:64], P.default_options(tolerance=tol, precision=prec, max_iterations=64 * 8,
                                                                 kernel_timing_reps=200, row_map=row_map,
-                                                                xcd_remap=xcd_remap))
+                                                                xcd_remap=xcd_remap, use_graph=graph))
     kern = {"k_col": (kst["col_kernel_bytes"], kst["col_kernel_ms"]),
             "k_row": (kst["row_kernel_bytes"], kst["row_kernel_ms"])}
     dom = max(kern, key=lambda k: kern[k][1])
@@ -336,6 +336,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every iteration's kernels directly instead of replaying hipGraphs of 64 iterations")
     ap.add_argument("--max-iterations", type=int, default=1_000_000,
                     help="cap per instance (the reference default, src/pdhg/pdhg.h:34); unconverged = unsolved")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -382,11 +384,12 @@ def main():
     lp = P.gen_refinery(SEED_LP)
     K = args.cases
     thetas, costs = P.gen_family(lp, SEED_THETA, K, 0.3)
+    graph = 0 if args.no_graph else 1
     opts = P.default_options(tolerance=args.tol, precision=prec, max_iterations=args.max_iterations,
                              progress_seconds=15.0, row_map=args.row_map, xcd_remap=args.xcd_remap,
-                             warm_fallback=args.warm_fallback)
+                             warm_fallback=args.warm_fallback, use_graph=graph)
     plain_opts = P.default_options(tolerance=args.tol, precision=prec, max_iterations=args.max_iterations,
-                                   row_map=args.row_map, xcd_remap=args.xcd_remap)
+                                   row_map=args.row_map, xcd_remap=args.xcd_remap, use_graph=graph)
     log(f"workload: refinery m={lp.m} n={lp.n} nnz={lp.nnz}, {K} instances, precision {args.precision}")
     if args.plain_all:
         pl = plain_gpu(lp, costs, plain_opts, K)
@@ -404,7 +407,7 @@ def main():
         fit_s = time.perf_counter() - t
         log(f"warm-start fit: {fit_s:.1f} s ({args.train} parameters at tolerance {args.train_tol})")
 
-    roof = kernel_roofline(lp, costs, prec, args.row_map, args.xcd_remap, args.tol)
+    roof = kernel_roofline(lp, costs, prec, args.row_map, args.xcd_remap, args.tol, graph)
     spmv = spmv_rates(lp)
     log("spmv (op-level k_spmv, scaled refinery matrix): " + ", ".join(
         f"{k} A x {v['Ax']['GBs']:.0f} / A^T y {v['ATy']['GBs']:.0f} GB/s" for k, v in spmv.items()))
