@@ -230,6 +230,50 @@ __device__ inline Acc row_dot_lds(const int* __restrict__ loc, const V* __restri
   return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
+// row_dot_lds for a row of len <= 64 entries whose tile columns and values
+// were staged in LDS (el[k], ev[k] for k < len) in the tile's load round,
+// with every operand read from LDS.  Entry k goes into accumulator k mod 4
+// in entry order, as in row_dot_wave; chunks of 4 keep the entries' columns
+// and values (now vector registers) within the occupancy budget, and the
+// predicated last chunk adds weight-0 terms only, so the sums are the same.
+template <typename Acc, typename X>
+__device__ inline Acc row_dot_lds_staged(const int* el, const Acc* ev, const X* tile, int len, int b) {
+  constexpr int CH = 4;
+  len = __builtin_amdgcn_readfirstlane(len);
+  Acc a[4] = {0, 0, 0, 0};
+  int k = 0;
+  for (; k + CH <= len; k += CH) {
+    int c[CH];
+    Acc v[CH], g[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      c[u] = el[k + u];
+      v[u] = ev[k + u];
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) g[u] = static_cast<Acc>(tile[c[u] * 64 + b]);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) a[u & 3] += v[u] * g[u];
+  }
+  if (k < len) {
+    const int last = len - 1;
+    int c[CH];
+    Acc v[CH], g[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int p = min(k + u, last);
+      c[u] = el[p];
+      const Acc w = ev[p];
+      v[u] = k + u <= last ? w : static_cast<Acc>(0);
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) g[u] = static_cast<Acc>(tile[c[u] * 64 + b]);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) a[u & 3] += v[u] * g[u];
+  }
+  return (a[0] + a[1]) + (a[2] + a[3]);
+}
+
 // Staging of tile t of a B = 64 plan in LDS: tile position u of lane b
 // holds the gathered row of the tile's u-th column for lane b (src may
 // differ per lane).  Wavefront w stages positions w, w + kTileWaves, ...

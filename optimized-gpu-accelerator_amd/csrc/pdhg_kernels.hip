@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
 // update operands; after one barrier the dot products read LDS.  loc holds
 // tile-local columns.
 template <typename T>
-__global__ __launch_bounds__(kTileWG) void k_col_tiled(RowPlan plan, const int* __restrict__ loc,
+__global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) void k_col_tiled(RowPlan plan, const int* __restrict__ loc,
                                                    const T* __restrict__ val, const T* __restrict__ y,
                                                    const T* __restrict__ yp, T* __restrict__ x, T* __restrict__ xp,
                                                    T* __restrict__ x0, const T* __restrict__ c, Ld lc,
@@ -263,25 +263,43 @@ __global__ __launch_bounds__(kTileWG) void k_col_tiled(RowPlan plan, const int* 
                                                    int tloc, double rho) {
   constexpr bool SPL = kSplit<T, 6>;
   __shared__ T tile[kTileCols<T> * 64];
+  __shared__ int entL[kTileWaves * 64];
+  __shared__ T entV[kTileWaves * 64];
   const int t = block_id(plan);
   const int b = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const TileRec<T> r = tile_record<T>(plan, t);
   const long long it = st.iter[b] + tloc;
   const bool rst = st.applyAt[b] == it;
+  // A row of at most 64 entries brings them in the same round of loads,
+  // one per lane, and reads them back from LDS.
+  const int len = r.e1 - r.e0;
+  const bool ent = len <= 64;
+  int eL = 0;
+  T eV = 0;
+  if (ent && b < len) {
+    eL = loc[r.e0 + b];
+    eV = val[r.e0 + b];
+  }
   T stage[kStageCols<T>];
   tile_load<SPL>(r, rst ? yp : y, b, stage);
   const int j = r.row;
   ColOps<T> ops;
   if (j >= 0) ops = col_load<T, 6>(j, b, rst, x, xp, x0, c, lc, lo, up, lb);
+  if (ent && b < len) {
+    entL[wave * 64 + b] = eL;
+    entV[wave * 64 + b] = eV;
+  }
   tile_store(r, stage, tile, b);
   if (j < 0) return;
-  const T acc = row_dot_lds<T>(loc, val, tile, r.e0, r.e1, b);
+  const T acc = ent ? row_dot_lds_staged<T>(entL + wave * 64, entV + wave * 64, tile, len, b)
+                    : row_dot_lds<T>(loc, val, tile, r.e0, r.e1, b);
   double q[2] = {0.0, 0.0};
   if (!st.done[b]) col_apply<T, 6, false>(j, acc, ops, b, rst, it, x, xp, x0, st, rho, q);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kTileWG) void k_row_tiled(RowPlan plan, const int* __restrict__ loc,
+__global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) void k_row_tiled(RowPlan plan, const int* __restrict__ loc,
                                                    const T* __restrict__ val, const T* __restrict__ xp,
                                                    T* __restrict__ y, T* __restrict__ yp, T* __restrict__ y0,
                                                    T* __restrict__ ax, T* __restrict__ axp, T* __restrict__ ax0,
@@ -291,19 +309,35 @@ __global__ __launch_bounds__(kTileWG) void k_row_tiled(RowPlan plan, const int* 
                                                    double rho) {
   constexpr bool SPL = kSplit<T, 6>;
   __shared__ T tile[kTileCols<T> * 64];
+  __shared__ int entL[kTileWaves * 64];
+  __shared__ T entV[kTileWaves * 64];
   const int t = block_id(plan);
   const int b = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const TileRec<T> r = tile_record<T>(plan, t);
   const long long it = st.iter[b] + tloc;
   const bool rst = st.applyAt[b] == it;
+  const int len = r.e1 - r.e0;
+  const bool ent = len <= 64;
+  int eL = 0;
+  T eV = 0;
+  if (ent && b < len) {
+    eL = loc[r.e0 + b];
+    eV = val[r.e0 + b];
+  }
   T stage[kStageCols<T>];
   tile_load<SPL>(r, xp, b, stage);
   const int i = r.row;
   RowOps<T> ops;
   if (i >= 0) ops = row_load<T, 6>(i, b, rst, y, yp, y0, ax, axp, ax0, rl, ru, lr);
+  if (ent && b < len) {
+    entL[wave * 64 + b] = eL;
+    entV[wave * 64 + b] = eV;
+  }
   tile_store(r, stage, tile, b);
   if (i < 0) return;
-  const T acc = row_dot_lds<T>(loc, val, tile, r.e0, r.e1, b);
+  const T acc = ent ? row_dot_lds_staged<T>(entL + wave * 64, entV + wave * 64, tile, len, b)
+                    : row_dot_lds<T>(loc, val, tile, r.e0, r.e1, b);
   double q[4] = {0.0, 0.0, 0.0, 0.0};
   if (!st.done[b])
     row_apply<T, 6, false>(i, acc, 0.0, ops, b, rst, it, y, yp, y0, ax, axp, ax0, rl64, ru64, lr, rfac, st, rho, q);
