@@ -385,6 +385,39 @@ def test_tiled_rows_match_one_row_per_wavefront(gpu, prec):
               f"iterations {st3['iterations']}")
 
 
+def test_graph_replays_equal_direct_launches(gpu):
+    """Blocks of check_every iterations replay hipGraphs, thousands of times a
+    solve, with the engine's own work between the replays: record fetches,
+    lanes loaded with new cases, compaction into narrower runs, the packed
+    runs' task-list rebuilds and hand-offs.  Every such run equals the same
+    run launched kernel by kernel, bit for bit: a width-64 batch on the
+    LDS-staged tiles, 96 cases streamed through a run of width 64, and a
+    packed batch of uneven LPs."""
+    lp = small_refinery()
+    _, costs = P.gen_family(lp, 9, 96, 0.3)
+    lps = P.gen_uneven_batch(7, 12, 400, 12000)
+
+    def same(ra, rb):
+        for a, b in zip(ra, rb):
+            assert a.status == b.status and a.iterations == b.iterations and a.objective == b.objective
+            assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+
+    out = {}
+    for graph in (1, 0):
+        fixed = dict(tolerance=1e-6, precision=P.FP64, use_graph=graph, xcd_remap=1)
+        out[graph] = (P.solve_batch(lp, costs[:64], P.default_options(row_map=3, **fixed)),
+                      P.solve_batch(lp, costs, P.default_options(row_map=1, lanes_per_row=4, **fixed)),
+                      P.solve_packed(lps, P.default_options(tolerance=1e-6, precision=P.FP64, use_graph=graph)))
+    (tiled_g, streamed_g, packed_g), (tiled_d, streamed_d, packed_d) = out[1], out[0]
+    assert tiled_g[1]["graph_launches"] > 0 and tiled_d[1]["graph_launches"] == 0
+    assert streamed_g[1]["graph_launches"] > 0 and tiled_g[1]["col_row_map"] == 3
+    same(tiled_g[0], tiled_d[0])
+    same(streamed_g[0], streamed_d[0])
+    same(packed_g[0], packed_d[0])
+    print(f"tiled batch {tiled_g[1]['graph_launches']} replays, streamed {streamed_g[1]['graph_launches']} replays, "
+          f"packed {packed_g[1]['graph_launches']} replays: equal to direct launches")
+
+
 def test_streamed_family_time_limit(gpu):
     """A run's time limit (a host-side clock, the reference's Deadline,
     src/util/timer.h:22-33) stops a continuous batch: converged cases keep
