@@ -123,8 +123,11 @@ class DeviceRun {
   // stream) and keeps the fastest per direction.  Runs on scratch state, so
   // it must come before upload() / init_state(); best[0] / best[1] get the
   // chosen candidate index for the column / row kernel, ms[0] / ms[1] its
-  // mean launch time.
-  virtual void autotune(const Prepared& P, const std::vector<MapSetup>& cand, int reps, int* best, double* ms) = 0;
+  // mean launch time, msHalf[0] / msHalf[1] the same with only the first
+  // half of the lanes live (lanes B/2..B-1 done), the state of a run into
+  // which half as many cases as lanes were re-packed.
+  virtual void autotune(const Prepared& P, const std::vector<MapSetup>& cand, int reps, int* best, double* ms,
+                        double* msHalf) = 0;
   // Whether the iterations without checks run in LDS-staged tiles
   // (MapSetup::tile), for the column (0) or row (1) kernel.
   virtual bool tiled(int side) const = 0;

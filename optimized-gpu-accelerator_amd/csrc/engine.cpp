@@ -94,7 +94,8 @@ struct TuneKey {
 // iteration (column + row kernel, ms) under it.
 struct Tuned {
   MapSetup col, row;
-  double ms = 0;
+  double ms = 0;    // column + row kernel, every lane live
+  double half = 0;  // the same with only the first half of the lanes live
 };
 
 std::mutex gTuneMutex;
@@ -348,11 +349,12 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       const auto t0 = Clock::now();
       const std::vector<MapSetup> cand = autoMap ? map_candidates(B) : std::vector<MapSetup>{fixed_map(B)};
       int best[2];
-      double ms[2];
-      run->autotune(P, cand, 8, best, ms);
+      double ms[2], msHalf[2];
+      run->autotune(P, cand, 8, best, ms, msHalf);
       t.col = cand[best[0]];
       t.row = cand[best[1]];
       t.ms = ms[0] + ms[1];
+      t.half = msHalf[0] + msHalf[1];
       remember(key, t);
       rs.colMap = t.col;
       rs.rowMap = t.row;
@@ -360,15 +362,16 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     }
     return std::make_pair(std::move(run), rs);
   };
-  // Time of one iteration at width W (probing the width once).
-  auto width_cost = [&](int W, int prec) {
+  // Time of one iteration at width W (probing the width once); packed: the
+  // live cases sit in the first half of the lanes.
+  auto width_cost = [&](int W, int prec, bool packed = false) {
     const TuneKey key{shape, W, prec == PDHG_FP64 ? PDHG_FP64 : PDHG_FP32, o.device, mapsKey};
     Tuned t;
     if (!tuned(key, &t)) {
       create_run(W, prec);
       tuned(key, &t);
     }
-    return t.ms;
+    return packed ? std::min(t.ms, t.half) : t.ms;
   };
   auto make_run = [&](const std::vector<int>& ids, int B) {
     auto [run, rs] = create_run(B, runPrec);
@@ -415,6 +418,9 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     if (k > 64)
       for (int q = k - 1; q >= 64; --q) pending.push_back(q);
     int B = next_pow2(static_cast<int>(ids.size()));
+    // Lanes 0..span-1 of the run may hold live cases: span < B after the
+    // live cases were re-packed into the first lanes of a run as wide.
+    int span = B;
     auto run = make_run(ids, B);
     transfer += since_s(tChunk);
     if (first == 0) setupSeconds = since_s(tSetup);
@@ -616,30 +622,37 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       // Re-pack: finished instances leave (compaction) and/or the run changes
       // precision; live instances keep their iterates, anchors and restart
       // state, moved through an FP64 host copy.
-      // Compaction: once at most half the lanes are live, move them to the
-      // width (a power of two holding them all) whose measured iteration is
-      // cheapest for this matrix and precision; a narrower run is not always
-      // cheaper (the row mapping changes with the width).
+      // Compaction: once at most half the lanes that may be live are, move
+      // the live cases into the first lanes of the run (a power of two wide,
+      // holding them all) whose measured iteration is cheapest for this
+      // matrix and precision.  A narrower run is not always cheaper (the row
+      // mapping changes with the width), and a run as wide as this one with
+      // its live cases packed into its first half can be cheaper still:
+      // lanes of finished cases load nothing, so whole cache lines of the
+      // vectors are skipped.
       const int newPrec = toFp64 ? PDHG_FP64 : runPrec;
       int newB = B;
-      const bool mayNarrow = o.compact && B > 1 && remaining <= B / 2 && pending.empty();
+      bool move = false;
+      const bool mayNarrow = o.compact && B > 1 && remaining <= span / 2 && pending.empty();
       if (!autoMap) {
         // A fixed mapping keeps runs reproducible: the width follows from the
         // live count alone (the narrowest power of two that holds them), not
         // from timings.
         if (mayNarrow) newB = next_pow2(remaining);
+        move = newB < B;
       } else if (mayNarrow || toFp64) {
-        double best = toFp64 ? kInf : width_cost(B, runPrec);
+        double best = toFp64 ? kInf : width_cost(B, runPrec, span < B);
         for (int W = next_pow2(remaining); W <= B; W *= 2) {
           if ((!o.compact || !pending.empty()) && W != B) continue;
-          const double c = width_cost(W, newPrec);
+          const double c = width_cost(W, newPrec, o.compact && pending.empty() && 2 * remaining <= W);
           if (c < best) {
             best = c;
             newB = W;
+            move = true;
           }
         }
       }
-      const bool compactNow = newB < B;
+      const bool compactNow = move;
       if (compactNow || toFp64) {
         std::vector<int> fin, live;
         for (int b = 0; b < real; ++b) (done[b] ? fin : live).push_back(b);
@@ -671,6 +684,10 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
           if (o.progress_seconds > 0)
             std::fprintf(stderr, "  pdhg batch: iteration %lld, FP32 progress stalled, continuing in FP64\n", it);
         }
+        span = next_pow2(static_cast<int>(ids.size()));
+        if (o.progress_seconds > 0)
+          std::fprintf(stderr, "  pdhg batch: iteration %lld, %d live moved into lanes 0..%d of a run of width %d\n",
+                       it, static_cast<int>(ids.size()), static_cast<int>(ids.size()) - 1, B);
         run = make_run(ids, B);
         run->import_lanes(ls);
         run->start(false);

@@ -308,11 +308,16 @@ __device__ inline TileRec<T> tile_record(const RowPlan& p, int t) {
 // tile_load issues a wavefront's share of the staging loads, tile_store
 // writes them to LDS; a kernel issues its other loads in between, then meets
 // the others at the barrier.
+// A lane whose instance is done (live false) loads nothing: once a run's live
+// instances sit in consecutive lanes, the cache lines of the finished half
+// of a staged row are not fetched at all.
 template <bool SPLIT, typename T>
-__device__ inline void tile_load(const TileRec<T>& r, const T* src, int b, T (&v)[kStageCols<T>]) {
+__device__ inline void tile_load(const TileRec<T>& r, const T* src, int b, bool live, T (&v)[kStageCols<T>]) {
 #pragma unroll
-  for (int k = 0; k < kStageCols<T>; ++k)
-    if (k < r.ns) v[k] = vld<SPLIT, 6>(src, r.col[k], b);
+  for (int k = 0; k < kStageCols<T>; ++k) {
+    v[k] = static_cast<T>(0);
+    if (k < r.ns && live) v[k] = vld<SPLIT, 6>(src, r.col[k], b);
+  }
 }
 
 template <typename T>

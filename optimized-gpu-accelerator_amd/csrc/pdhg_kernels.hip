@@ -271,6 +271,7 @@ __global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) vo
   const TileRec<T> r = tile_record<T>(plan, t);
   const long long it = st.iter[b] + tloc;
   const bool rst = st.applyAt[b] == it;
+  const bool live = !st.done[b];  // a finished instance's lane loads nothing
   // A row of at most 64 entries brings them in the same round of loads,
   // one per lane, and reads them back from LDS.
   const int len = r.e1 - r.e0;
@@ -282,10 +283,10 @@ __global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) vo
     eV = val[r.e0 + b];
   }
   T stage[kStageCols<T>];
-  tile_load<SPL>(r, rst ? yp : y, b, stage);
+  tile_load<SPL>(r, rst ? yp : y, b, live, stage);
   const int j = r.row;
   ColOps<T> ops;
-  if (j >= 0) ops = col_load<T, 6>(j, b, rst, x, xp, x0, c, lc, lo, up, lb);
+  if (j >= 0 && live) ops = col_load<T, 6>(j, b, rst, x, xp, x0, c, lc, lo, up, lb);
   if (ent && b < len) {
     entL[wave * 64 + b] = eL;
     entV[wave * 64 + b] = eV;
@@ -295,7 +296,7 @@ __global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) vo
   const T acc = ent ? row_dot_lds_staged<T>(entL + wave * 64, entV + wave * 64, tile, len, b)
                     : row_dot_lds<T>(loc, val, tile, r.e0, r.e1, b);
   double q[2] = {0.0, 0.0};
-  if (!st.done[b]) col_apply<T, 6, false>(j, acc, ops, b, rst, it, x, xp, x0, st, rho, q);
+  if (live) col_apply<T, 6, false>(j, acc, ops, b, rst, it, x, xp, x0, st, rho, q);
 }
 
 template <typename T>
@@ -317,6 +318,7 @@ __global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) vo
   const TileRec<T> r = tile_record<T>(plan, t);
   const long long it = st.iter[b] + tloc;
   const bool rst = st.applyAt[b] == it;
+  const bool live = !st.done[b];
   const int len = r.e1 - r.e0;
   const bool ent = len <= 64;
   int eL = 0;
@@ -326,10 +328,10 @@ __global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) vo
     eV = val[r.e0 + b];
   }
   T stage[kStageCols<T>];
-  tile_load<SPL>(r, xp, b, stage);
+  tile_load<SPL>(r, xp, b, live, stage);
   const int i = r.row;
   RowOps<T> ops;
-  if (i >= 0) ops = row_load<T, 6>(i, b, rst, y, yp, y0, ax, axp, ax0, rl, ru, lr);
+  if (i >= 0 && live) ops = row_load<T, 6>(i, b, rst, y, yp, y0, ax, axp, ax0, rl, ru, lr);
   if (ent && b < len) {
     entL[wave * 64 + b] = eL;
     entV[wave * 64 + b] = eV;
@@ -339,7 +341,7 @@ __global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) vo
   const T acc = ent ? row_dot_lds_staged<T>(entL + wave * 64, entV + wave * 64, tile, len, b)
                     : row_dot_lds<T>(loc, val, tile, r.e0, r.e1, b);
   double q[4] = {0.0, 0.0, 0.0, 0.0};
-  if (!st.done[b])
+  if (live)
     row_apply<T, 6, false>(i, acc, 0.0, ops, b, rst, it, y, yp, y0, ax, axp, ax0, rl64, ru64, lr, rfac, st, rho, q);
 }
 
@@ -1247,7 +1249,8 @@ class RunImpl final : public DeviceRun {
     copy_h2d(st_.done, full.data(), B_);
   }
 
-  void autotune(const Prepared& P, const std::vector<MapSetup>& cand, int reps, int* best, double* ms) override {
+  void autotune(const Prepared& P, const std::vector<MapSetup>& cand, int reps, int* best, double* ms,
+                double* msHalf) override {
     PDHG_REQUIRE(!cand.empty() && reps >= 1, "autotune: no candidates");
     // Scratch state: zero vectors, every lane live, no restart pending.
     const size_t nB = static_cast<size_t>(n_) * B_, mB = static_cast<size_t>(m_) * B_;
@@ -1303,6 +1306,24 @@ class RunImpl final : public DeviceRun {
       cur = h;
       best[dir] = bestI;
       ms[dir] = bestMs;
+      // The winner again with lanes B/2..B-1 done.
+      msHalf[dir] = bestMs;
+      if (B_ >= 2) {
+        std::vector<int> upper(B_, 0);
+        for (int b = B_ / 2; b < B_; ++b) upper[b] = 1;
+        copy_h2d(st_.done, upper.data(), B_);
+        auto one = [&] { dir == 0 ? launch_col(false, 0) : launch_row(false, 0); };
+        one();
+        HIP_OK(hipEventRecord(ev0_, stream_));
+        for (int r = 0; r < reps; ++r) one();
+        HIP_OK(hipEventRecord(ev1_, stream_));
+        HIP_OK(hipEventSynchronize(ev1_));
+        HIP_OK(hipGetLastError());
+        float el = 0;
+        HIP_OK(hipEventElapsedTime(&el, ev0_, ev1_));
+        msHalf[dir] = el / reps;
+        copy_h2d(st_.done, zeros.data(), B_);
+      }
     }
     colPart_ = buf_.alloc<double>(static_cast<size_t>(5) * B_ * colPlan_.blocks);
     rowPart_ = buf_.alloc<double>(static_cast<size_t>(4) * B_ * rowPlan_.blocks);
