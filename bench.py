@@ -116,6 +116,27 @@ def family_step(lp, thetas, costs, opts, ws=None, threads=16):
     return res, st, cert, ok
 
 
+def reference_check(res, cert, tol, k):
+    """The step's answers on the cases that the reference's own CPU PDHG (its
+    sources, FP64) solved ahead of time (tests/golden/c3_reference_cases.json,
+    the default workload only): objective error and certified dual bound
+    difference relative to the reference's, iteration counts apart."""
+    path = os.path.join(ROOT, "tests", "golden", "c3_reference_cases.json")
+    if tol != 1e-6 or k != 64 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        cases = json.load(f)["cases"]
+    obj, lb, its = [], [], []
+    for q, rec in cases.items():
+        r = res[int(q)]
+        obj.append(abs(r.objective - rec["objective"]) / (1 + abs(rec["objective"])))
+        lb.append(abs(cert[int(q), 0] - rec["safe_bound"]) / (1 + abs(rec["safe_bound"])))
+        its.append(abs(r.iterations - rec["iterations"]))
+    return {"cases": len(cases), "max_rel_objective_error": max(obj), "max_rel_bound_difference": max(lb),
+            "max_iteration_difference": int(max(its)),
+            "source": "tests/golden/c3_reference_cases.json (reference solvePdhg, its own sources, FP64)"}
+
+
 def spmv_rates(lp, reps=200):
     """Standalone CSR SpMV on the scaled matrix: the op-level backend's k_spmv
     (no atomics, rows bucketed by length; the reference's Backend::spmv and
@@ -490,7 +511,9 @@ def main():
         if not use_warm:
             step_s = [cold["seconds"]]  # the budget's step estimate follows the chosen start
 
-    certified, iters, kkt, gaps, inst_iters, fallbacks = 0, [], [], [], [], 0
+    certified, converged, iters, kkt, gaps, inst_iters, fallbacks = 0, 0, [], [], [], [], 0
+    capped_kkt = []
+    ref_check = None
     up_bytes = down_bytes = 0
     timed = []
     for s in range(args.steps):
@@ -504,13 +527,17 @@ def main():
         up_bytes += st["bytes_uploaded"]
         down_bytes += st["bytes_downloaded"]
         certified += int(ok.sum())
+        converged += sum(1 for r in res if r.status == P.OPTIMAL)
+        capped_kkt += [r.relative_kkt for r in res if r.status != P.OPTIMAL]
         iters.append(st["iterations"])
         fallbacks += st["warm_fallbacks"]
         inst_iters += [r.iterations for r in res]
-        kkt.append(max(r.relative_kkt for r in res))
+        kkt.append(max([r.relative_kkt for r in res if r.status == P.OPTIMAL] or [0.0]))
+        ref_check = reference_check(res, cert, args.tol, K)
         gaps.append(float(cert[:, 1].max()))
         log(f"step {s}: {timed[-1]:.2f} s, {st['iterations']} iterations, {ok.sum()}/{K} certified, "
-            f"{st['warm_fallbacks']} cold fallbacks, max rel-KKT {kkt[-1]:.2e}, max certified gap {gaps[-1]:.2e}")
+            f"{st['warm_fallbacks']} cold fallbacks, max rel-KKT of the converged {kkt[-1]:.2e}, "
+            f"max certified gap {gaps[-1]:.2e}")
     steps = len(timed)
     elapsed_timed = sum(timed)
     value = certified / elapsed_timed
@@ -563,7 +590,8 @@ def main():
                    "warm_start_training": (f"{args.train} family parameters at rel-KKT <= {args.train_tol:g}"
                                            if ws is not None else None),
                    "iterations": gpu_iters, "median_instance_iterations": float(np.median(inst_iters)),
-                   "certified": certified, "solved": steps * K, "cold_fallbacks": fallbacks},
+                   "cases_run": steps * K, "converged": converged, "certified": certified,
+                   "iteration_limit": steps * K - converged, "cold_fallbacks": fallbacks},
         "steps_requested": args.steps,
         "warmup_requested": args.warmup,
         "budget_s": args.budget,
@@ -583,7 +611,9 @@ def main():
         "packed_uneven": packed,
         "warm_start_fit_seconds": fit_s if ws is not None else None,
         "max_rel_kkt": max(kkt),
+        "rel_kkt_at_iteration_limit": capped_kkt,
         "max_certified_gap": max(gaps),
+        "reference_check": ref_check,
         "wall_s": elapsed(),
     }
     print(json.dumps(out), flush=True)

@@ -104,10 +104,11 @@ typedef struct pdhg_options {
   int xcd_remap;               /* -1 = auto, 0 = off, 1 = contiguous rows per XCD */
   double time_limit;           /* seconds, <= 0: none (Deadline, src/util/timer.h:22) */
   int device;                  /* HIP device ordinal (0) */
-  int kernel_timing_reps;      /* > 0: after the solve, time this many launches of each
-                                  iteration kernel with HIP events (stats *_kernel_ms) */
-  int compact;                 /* 1: once half the instances of a batch are done, move the
-                                  live ones into a run of half the width (default 1) */
+  int kernel_timing_reps;      /* > 0: after the solve, time this many iterations' column and
+                                  row kernels, each between HIP events (stats *_kernel_ms) */
+  int compact;                 /* 1: once half the lanes that may hold live instances are
+                                  live, move the live ones into the first lanes of the run
+                                  width measured cheapest (default 1) */
   double progress_seconds;     /* > 0: a progress line on stderr at this interval */
   long long warm_fallback;     /* cold-start fallback of warm-started cases (x0, y0 or
                                   primal_weight given): a case not converged after this

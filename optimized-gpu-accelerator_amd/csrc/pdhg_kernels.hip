@@ -1081,28 +1081,37 @@ class RunImpl final : public DeviceRun {
   }
 
   void time_kernels(int reps, double* msCol, double* msRow) override {
-    // Launches exactly as inside an iteration.  Called after the results were
-    // read: every instance is released (no done flag, no pending restart) so
-    // each launch streams and writes its full data.
+    // Launches exactly as inside an iteration, column and row kernel in
+    // turn, each between two events of its own.  Called after the results
+    // were read: every instance is released (no done flag, no pending
+    // restart) so each launch streams and writes its full data.
     std::vector<int> zeros(B_, 0);
     std::vector<long long> none(B_, -1);
     copy_h2d(st_.done, zeros.data(), B_);
     copy_h2d(st_.applyAt, none.data(), B_);
     launch_col(false, 0);
     launch_row(false, 0);
-    float ms = 0;
-    HIP_OK(hipEventRecord(ev0_, stream_));
-    for (int r = 0; r < reps; ++r) launch_col(false, 0);
-    HIP_OK(hipEventRecord(ev1_, stream_));
-    HIP_OK(hipEventSynchronize(ev1_));
-    HIP_OK(hipEventElapsedTime(&ms, ev0_, ev1_));
-    *msCol = ms / reps;
-    HIP_OK(hipEventRecord(ev0_, stream_));
-    for (int r = 0; r < reps; ++r) launch_row(false, 0);
-    HIP_OK(hipEventRecord(ev1_, stream_));
-    HIP_OK(hipEventSynchronize(ev1_));
-    HIP_OK(hipEventElapsedTime(&ms, ev0_, ev1_));
-    *msRow = ms / reps;
+    std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps) + 1);
+    for (auto& e : ev) HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipEventRecord(ev[0], stream_));
+    for (int r = 0; r < reps; ++r) {
+      launch_col(false, 0);
+      HIP_OK(hipEventRecord(ev[2 * r + 1], stream_));
+      launch_row(false, 0);
+      HIP_OK(hipEventRecord(ev[2 * r + 2], stream_));
+    }
+    HIP_OK(hipEventSynchronize(ev.back()));
+    double col = 0, row = 0;
+    for (int r = 0; r < reps; ++r) {
+      float a = 0, b = 0;
+      HIP_OK(hipEventElapsedTime(&a, ev[2 * r], ev[2 * r + 1]));
+      HIP_OK(hipEventElapsedTime(&b, ev[2 * r + 1], ev[2 * r + 2]));
+      col += a;
+      row += b;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *msCol = col / reps;
+    *msRow = row / reps;
   }
 
   void kernel_bytes(double* colBytes, double* rowBytes) const override {

@@ -10,13 +10,20 @@ K=64 family block of compareFamilyStrategies
 iterations, beyond what the FP64 oracle finishes in a test, so every answer
 is checked by size-independent properties: an FP64 host re-evaluation of the
 KKT residuals and the reference's certifier acceptance (implied bounds, safe
-dual bound, |obj - LB| <= 1e-4 (1 + |obj|)).
+dual bound, |obj - LB| <= 1e-4 (1 + |obj|)).  Six of the cases are also
+checked against the reference's own CPU PDHG (its sources, FP64), solved
+ahead of time into tests/golden/c3_reference_cases.json by
+tests/golden/make_c3_reference.py: iterations within 4 checks, objective and
+certified dual bound within 1e-6 relative.
 
 C4's uneven batch: 256 random LPs of the config-0 generator (seed 7, nnz
 log-uniform in [2e3, 2e5], a non-binding box x <= 2) in one packed run:
 each converged answer against its generator's closed-form optimum (relative
 to the size of the objective's terms), the host KKT re-evaluation and the
 certifier."""
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -24,6 +31,27 @@ import pdhg_amd as P
 from test_gpu_parity import host_kkt
 
 pytestmark = pytest.mark.gpu
+
+REFERENCE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_reference_cases.json")
+
+
+def check_against_reference(res, bounds, label):
+    """Iterations within 4 checks of the reference's (its test_solvers.cpp:128
+    bound), objective and certified dual bound within 1e-6 relative."""
+    with open(REFERENCE) as f:
+        cases = json.load(f)["cases"]
+    worst = [0.0, 0.0, 0]
+    for q, rec in cases.items():
+        r = res[int(q)]
+        assert r.status == P.OPTIMAL and rec["status"] == P.OPTIMAL, (label, q)
+        dobj = abs(r.objective - rec["objective"]) / (1 + abs(rec["objective"]))
+        dlb = abs(bounds[int(q)] - rec["safe_bound"]) / (1 + abs(rec["safe_bound"]))
+        dit = abs(r.iterations - rec["iterations"])
+        assert dit <= 4 * 64, (label, q, r.iterations, rec["iterations"])
+        assert dobj <= 1e-6 and dlb <= 1e-6, (label, q, dobj, dlb)
+        worst = [max(worst[0], dobj), max(worst[1], dlb), max(worst[2], dit)]
+    print(f"{label} against the reference's CPU PDHG on {len(cases)} cases: objective {worst[0]:.1e}, "
+          f"bound {worst[1]:.1e}, iterations {worst[2]} apart at most")
 
 
 @pytest.fixture(scope="module")
@@ -60,6 +88,7 @@ def test_c3_family_cold_batch(gpu, family):
           f"iterations, median case {np.median(its):.0f}, {st['solve_seconds']:.1f} s, tuned col_row_map "
           f"{st['col_row_map']} row_row_map {st['row_row_map']}")
     assert st["batch"] == 64 and st["autotuned"] == 1 and st["graph_launches"] > 0
+    check_against_reference(res, cert[:, 0], "C3 cold")
     # The hardest cases of this ill-conditioned family need about the
     # reference's 1M-iteration cap: at most two stop there.
     assert len(conv) >= 62
