@@ -301,13 +301,17 @@ def kernel_roofline(lp, costs, prec, row_map, xcd_remap, tol, graph=1):
     tf = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get(f"{dom}{'_tiled' if tiled else ''}_{kprec}")
+    ibytes, ims = kern["k_col"][0] + kern["k_row"][0], kst["iter_kernel_ms"]
     log(f"kernels: k_col {kern['k_col'][1] * 1e3:.1f} us ({kern['k_col'][0] / kern['k_col'][1] / 1e6:.0f} GB/s), "
-        f"k_row {kern['k_row'][1] * 1e3:.1f} us ({kern['k_row'][0] / kern['k_row'][1] / 1e6:.0f} GB/s)")
+        f"k_row {kern['k_row'][1] * 1e3:.1f} us ({kern['k_row'][0] / kern['k_row'][1] / 1e6:.0f} GB/s), "
+        f"iteration {ims * 1e3:.1f} us ({ibytes / ims / 1e6:.0f} GB/s)")
     return {"bound": "hbm", "kernel": dom + ("_tiled" if tiled else ""), "achieved": achieved, "peak": PEAK_HBM_GBS,
             "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "algorithmic_bytes": dbytes,
             "launch_us": dms * 1e3, "xcd_remap": kst["col_xcd" if dom == "k_col" else "row_xcd"],
             "other_kernel": {k: {"launch_us": v[1] * 1e3, "algorithmic_bytes": v[0]} for k, v in kern.items()
-                             if k != dom}}
+                             if k != dom},
+            "iteration": {"kernels": "k_col then k_row, in turn", "us": ims * 1e3, "algorithmic_bytes": ibytes,
+                          "achieved": ibytes / (ims * 1e-3) / 1e9, "frac": ibytes / (ims * 1e-3) / 1e9 / PEAK_HBM_GBS}}
 
 
 def plain_gpu(lp, costs, opts, sample):

@@ -104,8 +104,9 @@ typedef struct pdhg_options {
   int xcd_remap;               /* -1 = auto, 0 = off, 1 = contiguous rows per XCD */
   double time_limit;           /* seconds, <= 0: none (Deadline, src/util/timer.h:22) */
   int device;                  /* HIP device ordinal (0) */
-  int kernel_timing_reps;      /* > 0: after the solve, time this many iterations' column and
-                                  row kernels, each between HIP events (stats *_kernel_ms) */
+  int kernel_timing_reps;      /* > 0: after the solve, time this many launches of each
+                                  iteration kernel, then this many iterations, with HIP
+                                  events (stats *_kernel_ms) */
   int compact;                 /* 1: once half the lanes that may hold live instances are
                                   live, move the live ones into the first lanes of the run
                                   width measured cheapest (default 1) */
@@ -159,6 +160,8 @@ typedef struct pdhg_stats {
   double tune_seconds;
   int col_rows_per_wave, row_rows_per_wave; /* B = 64: 1, or 2 rows per wavefront */
   int warm_fallbacks;          /* warm-started cases restarted from the cold start */
+  double iter_kernel_ms;       /* mean iteration, column and row kernel in turn
+                                  (kernel_timing_reps > 0) */
 } pdhg_stats;
 
 /* PdhgResult (src/pdhg/pdhg.h:58-65).  x (n) and row_dual (m) are

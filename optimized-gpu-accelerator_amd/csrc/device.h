@@ -97,9 +97,10 @@ class DeviceRun {
   virtual void download(int which, double* out) = 0;
   // Stops the loop timer: device seconds since start().
   virtual double stop() = 0;
-  // Mean duration (ms) of the column and row iteration kernels, launched in
-  // turn as in `reps` iterations, each between HIP events on the run's stream.
-  virtual void time_kernels(int reps, double* msCol, double* msRow) = 0;
+  // Mean duration (ms) of the column and of the row iteration kernel, each
+  // over `reps` back-to-back launches, and of one iteration (the two in
+  // turn) over `reps` iterations, HIP events on the run's stream.
+  virtual void time_kernels(int reps, double* msCol, double* msRow, double* msIter) = 0;
   // Algorithmic bytes of one launch of the column and row iteration kernels.
   virtual void kernel_bytes(double* colBytes, double* rowBytes) const = 0;
   // Copies the state of `lanes` (in that order) to the host.

@@ -712,7 +712,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     if (o.kernel_timing_reps > 0 && !timed) {
       // After the results were read, every instance is released so each
       // timed launch does the full work of an iteration.
-      run->time_kernels(o.kernel_timing_reps, &acc.col_kernel_ms, &acc.row_kernel_ms);
+      run->time_kernels(o.kernel_timing_reps, &acc.col_kernel_ms, &acc.row_kernel_ms, &acc.iter_kernel_ms);
       double colB = 0, rowB = 0;
       run->kernel_bytes(&colB, &rowB);
       acc.col_kernel_bytes = colB;

@@ -1080,38 +1080,33 @@ class RunImpl final : public DeviceRun {
     return ms / 1000.0;
   }
 
-  void time_kernels(int reps, double* msCol, double* msRow) override {
-    // Launches exactly as inside an iteration, column and row kernel in
-    // turn, each between two events of its own.  Called after the results
-    // were read: every instance is released (no done flag, no pending
-    // restart) so each launch streams and writes its full data.
+  void time_kernels(int reps, double* msCol, double* msRow, double* msIter) override {
+    // Called after the results were read: every instance is released (no
+    // done flag, no pending restart) so each launch streams and writes its
+    // full data.  Each kernel alone over `reps` back-to-back launches, then
+    // `reps` iterations (column and row kernel in turn), HIP events at the
+    // ends of each series only.
     std::vector<int> zeros(B_, 0);
     std::vector<long long> none(B_, -1);
     copy_h2d(st_.done, zeros.data(), B_);
     copy_h2d(st_.applyAt, none.data(), B_);
     launch_col(false, 0);
     launch_row(false, 0);
-    std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps) + 1);
-    for (auto& e : ev) HIP_OK(hipEventCreate(&e));
-    HIP_OK(hipEventRecord(ev[0], stream_));
-    for (int r = 0; r < reps; ++r) {
-      launch_col(false, 0);
-      HIP_OK(hipEventRecord(ev[2 * r + 1], stream_));
-      launch_row(false, 0);
-      HIP_OK(hipEventRecord(ev[2 * r + 2], stream_));
-    }
-    HIP_OK(hipEventSynchronize(ev.back()));
-    double col = 0, row = 0;
-    for (int r = 0; r < reps; ++r) {
-      float a = 0, b = 0;
-      HIP_OK(hipEventElapsedTime(&a, ev[2 * r], ev[2 * r + 1]));
-      HIP_OK(hipEventElapsedTime(&b, ev[2 * r + 1], ev[2 * r + 2]));
-      col += a;
-      row += b;
-    }
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    *msCol = col / reps;
-    *msRow = row / reps;
+    auto series = [&](bool col, bool row) {
+      HIP_OK(hipEventRecord(ev0_, stream_));
+      for (int r = 0; r < reps; ++r) {
+        if (col) launch_col(false, 0);
+        if (row) launch_row(false, 0);
+      }
+      HIP_OK(hipEventRecord(ev1_, stream_));
+      HIP_OK(hipEventSynchronize(ev1_));
+      float ms = 0;
+      HIP_OK(hipEventElapsedTime(&ms, ev0_, ev1_));
+      return static_cast<double>(ms) / reps;
+    };
+    *msCol = series(true, false);
+    *msRow = series(false, true);
+    *msIter = series(true, true);
   }
 
   void kernel_bytes(double* colBytes, double* rowBytes) const override {

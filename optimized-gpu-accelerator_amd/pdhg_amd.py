@@ -65,7 +65,7 @@ class Stats(C.Structure):
                 ("autotuned", C.c_int), ("col_nnz_per_lane", C.c_int), ("col_row_map", C.c_int), ("col_xcd", C.c_int),
                 ("row_nnz_per_lane", C.c_int), ("row_row_map", C.c_int), ("row_xcd", C.c_int),
                 ("tune_seconds", C.c_double), ("col_rows_per_wave", C.c_int), ("row_rows_per_wave", C.c_int),
-                ("warm_fallbacks", C.c_int)]
+                ("warm_fallbacks", C.c_int), ("iter_kernel_ms", C.c_double)]
 
 
 class ResultStruct(C.Structure):

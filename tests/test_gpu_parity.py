@@ -308,28 +308,38 @@ def test_lognormal_lp_full_size(gpu, prec, tol):
 
 def test_refinery_lp_full_size(gpu):
     """The full refinery LP (config C2: 555,852 nonzeros, rows/columns scaled
-    by 10^U(-3,3)), single instance: at rel-KKT 1e-4 with FP32 iterates (host
-    re-evaluation of the residuals), and at 1e-6 with FP64 and with FP32_AUTO
-    iterates, both accepted by the certifier and agreeing to its level.  Each
-    objective is compared with the reference's own CPU PDHG on the same LP and
-    tolerance (tests/golden/c3_reference_cases.json, "base", written by
-    tests/golden/make_c3_reference.py): FP64 within 1e-6 relative and 4
-    checks of its iteration count; an FP32 answer within what two answers at
-    that KKT tolerance may differ by (a few times the tolerance, relative)."""
+    by 10^U(-3,3)), single instance, against the reference's own CPU PDHG on
+    the same LP and tolerance (tests/golden/c3_reference_cases.json, "base",
+    written by tests/golden/make_c3_reference.py).
+    * FP64 iterates at rel-KKT 1e-4 and 1e-6: the reference's trajectory, so
+      its iteration count within 4 checks and its objective within 1e-6.
+    * FP32 iterates at 1e-4 (host re-evaluation of the residuals) and
+      FP32_AUTO at 1e-6 (certified): another trajectory, stopped at another
+      point that meets the same tolerance.  Relative KKT bounds the gap and
+      the residuals, not the distance of the objective from the reference's
+      answer: a primal residual within 1e-4 moves the objective by up to the
+      duals times that residual.  Measured: 1.3e-3 at 1e-4 (the reference's
+      own 1e-4 answer is 6e-5 from the 1e-6 optimum), so the bounds are 5e-3
+      at 1e-4 and 1e-4 at 1e-6."""
     import json
 
     with open(Path(__file__).parent / "golden" / "c3_reference_cases.json") as f:
         ref = json.load(f)["base"]
     lp = P.gen_refinery(3)
-    r, st, _ = P.solve(lp, P.default_options(tolerance=1e-4, precision=P.FP32))
     rr = ref["0.0001"]
+    r, st, _ = P.solve(lp, P.default_options(tolerance=1e-4, precision=P.FP64))
+    err = abs(r.objective - rr["objective"]) / (1 + abs(rr["objective"]))
+    print(f"refinery 1e-4 FP64: {r.iterations} iterations (reference {rr['iterations']}), objective against the "
+          f"reference's {err:.1e}")
+    assert r.status == P.OPTIMAL and err <= 1e-6 and abs(r.iterations - rr["iterations"]) <= 4 * 64
+    r, st, _ = P.solve(lp, P.default_options(tolerance=1e-4, precision=P.FP32))
     err = abs(r.objective - rr["objective"]) / (1 + abs(rr["objective"]))
     print(f"refinery 1e-4 FP32: {r.iterations} iterations, {st['solve_seconds']:.2f} s, objective against the "
           f"reference's {err:.1e}")
     assert r.status == P.OPTIMAL and r.relative_kkt <= 1e-4
     relP, relD, gap, _ = host_kkt(lp, r.x, r.y)
     assert max(relP, relD, gap) <= 1.5e-4, (relP, relD, gap)
-    assert err <= 5e-4
+    assert err <= 5e-3
     out = {}
     for prec in (P.FP64, P.FP32_AUTO):
         r, st, _ = P.solve(lp, P.default_options(tolerance=1e-6, precision=prec))
@@ -346,7 +356,7 @@ def test_refinery_lp_full_size(gpu):
         if prec == P.FP64:
             assert err <= 1e-6 and abs(r.iterations - rr["iterations"]) <= 4 * 64
         else:
-            assert err <= 1e-5
+            assert err <= 1e-4
         out[prec] = r.objective
     assert abs(out[P.FP32_AUTO] - out[P.FP64]) <= 1e-4 * (1 + abs(out[P.FP64]))
 
