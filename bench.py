@@ -296,16 +296,18 @@ def kernel_roofline(lp, costs, prec, row_map, xcd_remap, tol, graph=1):
     dbytes, dms = kern[dom]
     achieved = dbytes / (dms * 1e-3) / 1e9
     kprec = "fp64" if prec == P.FP64 else "fp32"
-    tiled = kst["col_row_map" if dom == "k_col" else "row_row_map"] == 3  # LDS-staged tiles
+    rmap = kst["col_row_map" if dom == "k_col" else "row_row_map"]
+    # LDS-staged tiles of 16, 8 or 4 rows: k_col_tiled, k_col_tiled8, k_col_tiled4
+    name = dom + {3: "_tiled", 4: "_tiled8", 5: "_tiled4"}.get(rmap, "")
     traffic = None
     tf = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get(f"{dom}{'_tiled' if tiled else ''}_{kprec}")
+        traffic = json.load(open(tf)).get(f"{name}_{kprec}")
     ibytes, ims = kern["k_col"][0] + kern["k_row"][0], kst["iter_kernel_ms"]
     log(f"kernels: k_col {kern['k_col'][1] * 1e3:.1f} us ({kern['k_col'][0] / kern['k_col'][1] / 1e6:.0f} GB/s), "
         f"k_row {kern['k_row'][1] * 1e3:.1f} us ({kern['k_row'][0] / kern['k_row'][1] / 1e6:.0f} GB/s), "
         f"iteration {ims * 1e3:.1f} us ({ibytes / ims / 1e6:.0f} GB/s)")
-    return {"bound": "hbm", "kernel": dom + ("_tiled" if tiled else ""), "achieved": achieved, "peak": PEAK_HBM_GBS,
+    return {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": PEAK_HBM_GBS,
             "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "algorithmic_bytes": dbytes,
             "launch_us": dms * 1e3, "xcd_remap": kst["col_xcd" if dom == "k_col" else "row_xcd"],
             "other_kernel": {k: {"launch_us": v[1] * 1e3, "algorithmic_bytes": v[0]} for k, v in kern.items()

@@ -100,7 +100,8 @@ typedef struct pdhg_options {
   int use_graph;               /* capture check_every iterations in a hipGraph (1) */
   int lanes_per_row;           /* 0 = autotune; else nnz-per-lane target (1..64) */
   int row_map;                 /* 0 = auto, 1 = row-length buckets, 2 = one mapping,
-                                  3 = LDS-staged row tiles (width 64; else as 1) */
+                                  3, 4, 5 = LDS-staged row tiles of 16, 8, 4 rows
+                                  (width 64; else as 1) */
   int xcd_remap;               /* -1 = auto, 0 = off, 1 = contiguous rows per XCD */
   double time_limit;           /* seconds, <= 0: none (Deadline, src/util/timer.h:22) */
   int device;                  /* HIP device ordinal (0) */
@@ -151,7 +152,7 @@ typedef struct pdhg_stats {
    * (column kernel over A^T, row kernel over A): nnz-per-lane target (0 =
    * one row per B-lane group, rows' entries read through the scalar cache),
    * 1 = row-length buckets (grouped form: rows ordered by length) / 2 = one
-   * mapping, XCD remap on/off.  autotuned = 1
+   * mapping / 3, 4, 5 = LDS-staged tiles of 16, 8, 4 rows, XCD remap on/off.  autotuned = 1
    * when the options left all three on auto and the autotuner chose them
    * (timed on the device at the run's width, cached per matrix and width). */
   int autotuned;

@@ -39,8 +39,9 @@ struct MapSetup {
   // B = 64: rows per wavefront (1, or 2 with their chunks interleaved).
   int rowsPerWave = 1;
   // B = 64, iterations without checks: rows in locality order, grouped in
-  // tiles whose distinct gathered rows a workgroup stages in LDS once.
-  bool tile = false;
+  // tiles whose distinct gathered rows a workgroup stages in LDS once
+  // (0: no tiles; else the tile shape of tiles.h, 1..kTileShapes).
+  int tile = 0;
 };
 
 struct RunSetup {
@@ -119,19 +120,21 @@ class DeviceRun {
                           const double* cNorm, const double* rhsNorm) = 0;
   // Stops lanes (their kernels skip them from the next iteration on).
   virtual void retire_lanes(const std::vector<int>& lanes) = 0;
-  // Autotuner: times the column and the row iteration kernel under each
-  // candidate row mapping (`reps` launches each, HIP events on the run's
-  // stream) and keeps the fastest per direction.  Runs on scratch state, so
-  // it must come before upload() / init_state(); best[0] / best[1] get the
-  // chosen candidate index for the column / row kernel, ms[0] / ms[1] its
-  // mean launch time, msHalf[0] / msHalf[1] the same with only the first
-  // half of the lanes live (lanes B/2..B-1 done), the state of a run into
-  // which half as many cases as lanes were re-packed.
-  virtual void autotune(const Prepared& P, const std::vector<MapSetup>& cand, int reps, int* best, double* ms,
-                        double* msHalf) = 0;
-  // Whether the iterations without checks run in LDS-staged tiles
-  // (MapSetup::tile), for the column (0) or row (1) kernel.
-  virtual bool tiled(int side) const = 0;
+  // Autotuner: times iterations (column and row kernel in turn, `reps` of
+  // them, HIP events on the run's stream) with each candidate row mapping,
+  // first for the column kernel (with the run's row mapping), then for the
+  // row kernel (with the chosen column mapping), and keeps the fastest per
+  // direction.  Runs on scratch state, so it must come before upload() /
+  // init_state(); best[0] / best[1] get the chosen candidate index for the
+  // column / row kernel, msIter the mean iteration under both choices and
+  // msIterHalf the same with only the first half of the lanes live (lanes
+  // B/2..B-1 done), the state of a run into which half as many cases as
+  // lanes were re-packed.
+  virtual void autotune(const Prepared& P, const std::vector<MapSetup>& cand, int reps, int* best, double* msIter,
+                        double* msIterHalf) = 0;
+  // The tile shape (MapSetup::tile) in which the iterations without checks
+  // run, 0 without tiles, for the column (0) or row (1) kernel.
+  virtual int tiled(int side) const = 0;
 
   long long launches = 0, graphLaunches = 0, bytesUp = 0, bytesDown = 0;
 };

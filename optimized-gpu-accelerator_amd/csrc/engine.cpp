@@ -10,6 +10,7 @@
 #include <tuple>
 
 #include "device.h"
+#include "tiles.h"
 #include "prepare.h"
 
 namespace pdhgamd {
@@ -94,7 +95,7 @@ struct TuneKey {
 // iteration (column + row kernel, ms) under it.
 struct Tuned {
   MapSetup col, row;
-  double ms = 0;    // column + row kernel, every lane live
+  double ms = 0;    // one iteration (column + row kernel), every lane live
   double half = 0;  // the same with only the first half of the lanes live
 };
 
@@ -146,7 +147,8 @@ std::vector<MapSetup> map_candidates(int B) {
     if (!xcd && B >= 16) continue;
     if (B == 64) {
       for (int rpw : {1, 2}) c.push_back(MapSetup{4, 1, xcd, false, rpw});
-      c.push_back(MapSetup{4, 1, xcd, false, 1, true});  // rows in LDS-staged tiles
+      for (int shape = 1; shape <= kTileShapes; ++shape)
+        c.push_back(MapSetup{4, 1, xcd, false, 1, shape});  // rows in LDS-staged tiles
       continue;
     }
     for (int per : {2, 4, 8, 16})
@@ -168,7 +170,7 @@ void check_options(const pdhg_options& o) {
   PDHG_REQUIRE(o.precision == PDHG_FP32 || o.precision == PDHG_FP64 || o.precision == PDHG_FP32_AUTO,
                "unknown precision");
   PDHG_REQUIRE(o.lanes_per_row >= 0 && o.lanes_per_row <= 64, "lanes_per_row must be in 0..64");
-  PDHG_REQUIRE(o.row_map >= 0 && o.row_map <= 3, "row_map must be in 0..3");
+  PDHG_REQUIRE(o.row_map >= 0 && o.row_map <= 2 + kTileShapes, "row_map must be in 0..5");
   PDHG_REQUIRE(o.xcd_remap >= -1 && o.xcd_remap <= 1, "xcd_remap must be -1, 0 or 1");
   PDHG_REQUIRE(o.device >= 0 && o.device < kMaxDevices, "device ordinal out of range");
   PDHG_REQUIRE(o.warm_fallback >= -1, "warm_fallback must be >= -1");
@@ -322,7 +324,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     MapSetup ms;
     ms.nnzPerLane = o.lanes_per_row > 0 ? o.lanes_per_row : 4;
     ms.rowMap = o.row_map == 2 ? 2 : 1;
-    ms.tile = o.row_map == 3 && B == 64;
+    ms.tile = B == 64 && o.row_map >= 3 ? o.row_map - 2 : 0;  // 3, 4, 5: tile shapes 1, 2, 3
     ms.xcd = o.xcd_remap == 1 || (o.xcd_remap < 0 && B >= 16);
     return ms;
   };
@@ -349,12 +351,9 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       const auto t0 = Clock::now();
       const std::vector<MapSetup> cand = autoMap ? map_candidates(B) : std::vector<MapSetup>{fixed_map(B)};
       int best[2];
-      double ms[2], msHalf[2];
-      run->autotune(P, cand, 8, best, ms, msHalf);
+      run->autotune(P, cand, 16, best, &t.ms, &t.half);
       t.col = cand[best[0]];
       t.row = cand[best[1]];
-      t.ms = ms[0] + ms[1];
-      t.half = msHalf[0] + msHalf[1];
       remember(key, t);
       rs.colMap = t.col;
       rs.rowMap = t.row;
@@ -742,10 +741,10 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
     std::snprintf(acc.device_name, sizeof(acc.device_name), "%s", di.name.c_str());
     acc.autotuned = tunedFirst ? 1 : 0;
     acc.col_nnz_per_lane = firstCol.group ? 0 : firstCol.nnzPerLane;
-    acc.col_row_map = firstCol.tile ? 3 : firstCol.rowMap;
+    acc.col_row_map = firstCol.tile ? 2 + firstCol.tile : firstCol.rowMap;
     acc.col_xcd = firstCol.xcd ? 1 : 0;
     acc.row_nnz_per_lane = firstRow.group ? 0 : firstRow.nnzPerLane;
-    acc.row_row_map = firstRow.tile ? 3 : firstRow.rowMap;
+    acc.row_row_map = firstRow.tile ? 2 + firstRow.tile : firstRow.rowMap;
     acc.row_xcd = firstRow.xcd ? 1 : 0;
     acc.tune_seconds = tuneSeconds;
     acc.col_rows_per_wave = firstCol.rowsPerWave;

@@ -276,32 +276,31 @@ __device__ inline Acc row_dot_lds_staged(const int* el, const Acc* ev, const X* 
 
 // Staging of tile t of a B = 64 plan in LDS: tile position u of lane b
 // holds the gathered row of the tile's u-th column for lane b (src may
-// differ per lane).  Wavefront w stages positions w, w + kTileWaves, ...
-// Its record (tile_descriptors) comes in one round of scalar loads with no
-// load before it, so the gathers start one memory latency after launch.
-template <typename T>
-constexpr int kStageCols = (kTileCols<T> + kTileWaves - 1) / kTileWaves;  // per wavefront at most
-template <typename T>
-constexpr int kDescInts = desc_stride(kStageCols<T>);
+// differ per lane).  Of a tile shape's WAVES wavefronts, wavefront w stages
+// positions w, w + WAVES, ...; NS positions at most.  Its record
+// (tile_descriptors) comes in one round of scalar loads with no load before
+// it, so the gathers start one memory latency after launch.
+template <typename T, int SHAPE>
+constexpr int kStageCols = (kTileCols<T, SHAPE> + tile_waves(SHAPE) - 1) / tile_waves(SHAPE);
 
 // The wavefront's record of tile t, read into registers (wave-uniform).
-template <typename T>
+template <int NS>
 struct TileRec {
   int row, e0, e1, ns;
-  int col[kStageCols<T>];
+  int col[NS];
 };
 
-template <typename T>
-__device__ inline TileRec<T> tile_record(const RowPlan& p, int t) {
+template <int NS, int WAVES>
+__device__ inline TileRec<NS> tile_record(const RowPlan& p, int t) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  const int* __restrict__ d = p.desc + (static_cast<size_t>(t) * kTileWaves + wave) * kDescInts<T>;
-  TileRec<T> r;
+  const int* __restrict__ d = p.desc + (static_cast<size_t>(t) * WAVES + wave) * desc_stride(NS);
+  TileRec<NS> r;
   r.row = d[0];
   r.e0 = d[1];
   r.e1 = d[2];
   r.ns = d[3];
 #pragma unroll
-  for (int k = 0; k < kStageCols<T>; ++k) r.col[k] = d[4 + k];
+  for (int k = 0; k < NS; ++k) r.col[k] = d[4 + k];
   return r;
 }
 
@@ -311,21 +310,21 @@ __device__ inline TileRec<T> tile_record(const RowPlan& p, int t) {
 // A lane whose instance is done (live false) loads nothing: once a run's live
 // instances sit in consecutive lanes, the cache lines of the finished half
 // of a staged row are not fetched at all.
-template <bool SPLIT, typename T>
-__device__ inline void tile_load(const TileRec<T>& r, const T* src, int b, bool live, T (&v)[kStageCols<T>]) {
+template <bool SPLIT, typename T, int NS>
+__device__ inline void tile_load(const TileRec<NS>& r, const T* src, int b, bool live, T (&v)[NS]) {
 #pragma unroll
-  for (int k = 0; k < kStageCols<T>; ++k) {
+  for (int k = 0; k < NS; ++k) {
     v[k] = static_cast<T>(0);
     if (k < r.ns && live) v[k] = vld<SPLIT, 6>(src, r.col[k], b);
   }
 }
 
-template <typename T>
-__device__ inline void tile_store(const TileRec<T>& r, const T (&v)[kStageCols<T>], T* tile, int b) {
+template <int WAVES, typename T, int NS>
+__device__ inline void tile_store(const TileRec<NS>& r, const T (&v)[NS], T* tile, int b) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
 #pragma unroll
-  for (int k = 0; k < kStageCols<T>; ++k)
-    if (k < r.ns) tile[(wave + k * kTileWaves) * 64 + b] = v[k];
+  for (int k = 0; k < NS; ++k)
+    if (k < r.ns) tile[(wave + k * WAVES) * 64 + b] = v[k];
   __syncthreads();
 }
 

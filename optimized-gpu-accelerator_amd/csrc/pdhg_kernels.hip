@@ -248,27 +248,29 @@ __global__ __launch_bounds__(kWG) void k_row(RowPlan plan, const int* __restrict
 }
 
 // B = 64 tiled forms of k_col / k_row (no check quantities): a workgroup
-// takes one tile (up to kTileWaves rows in locality order), each wavefront
-// one row of it.  The wavefront's record (row, entry range, staged columns)
+// takes one tile (up to tile_waves(SHAPE) rows in locality order), each
+// wavefront one row of it.  The wavefront's record (row, entry range, staged columns)
 // arrives in one round of scalar loads; one round of vector loads then
 // brings the tile's distinct gathered rows (staged in LDS) and the row's
 // update operands; after one barrier the dot products read LDS.  loc holds
 // tile-local columns.
-template <typename T>
-__global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) void k_col_tiled(RowPlan plan, const int* __restrict__ loc,
+template <typename T, int SHAPE>
+__global__ __launch_bounds__(64 * tile_waves(SHAPE)) __attribute__((amdgpu_waves_per_eu(8))) void k_col_tiled(RowPlan plan, const int* __restrict__ loc,
                                                    const T* __restrict__ val, const T* __restrict__ y,
                                                    const T* __restrict__ yp, T* __restrict__ x, T* __restrict__ xp,
                                                    T* __restrict__ x0, const T* __restrict__ c, Ld lc,
                                                    const T* __restrict__ lo, const T* __restrict__ up, Ld lb, Inst st,
                                                    int tloc, double rho) {
   constexpr bool SPL = kSplit<T, 6>;
-  __shared__ T tile[kTileCols<T> * 64];
-  __shared__ int entL[kTileWaves * 64];
-  __shared__ T entV[kTileWaves * 64];
+  constexpr int WAVES = tile_waves(SHAPE);
+  constexpr int NS = kStageCols<T, SHAPE>;
+  __shared__ T tile[kTileCols<T, SHAPE> * 64];
+  __shared__ int entL[WAVES * 64];
+  __shared__ T entV[WAVES * 64];
   const int t = block_id(plan);
   const int b = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  const TileRec<T> r = tile_record<T>(plan, t);
+  const TileRec<NS> r = tile_record<NS, WAVES>(plan, t);
   const long long it = st.iter[b] + tloc;
   const bool rst = st.applyAt[b] == it;
   const bool live = !st.done[b];  // a finished instance's lane loads nothing
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) vo
     eL = loc[r.e0 + b];
     eV = val[r.e0 + b];
   }
-  T stage[kStageCols<T>];
+  T stage[NS];
   tile_load<SPL>(r, rst ? yp : y, b, live, stage);
   const int j = r.row;
   ColOps<T> ops;
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) vo
     entL[wave * 64 + b] = eL;
     entV[wave * 64 + b] = eV;
   }
-  tile_store(r, stage, tile, b);
+  tile_store<WAVES>(r, stage, tile, b);
   if (j < 0) return;
   const T acc = ent ? row_dot_lds_staged<T>(entL + wave * 64, entV + wave * 64, tile, len, b)
                     : row_dot_lds<T>(loc, val, tile, r.e0, r.e1, b);
@@ -299,8 +301,8 @@ __global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) vo
   if (live) col_apply<T, 6, false>(j, acc, ops, b, rst, it, x, xp, x0, st, rho, q);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) void k_row_tiled(RowPlan plan, const int* __restrict__ loc,
+template <typename T, int SHAPE>
+__global__ __launch_bounds__(64 * tile_waves(SHAPE)) __attribute__((amdgpu_waves_per_eu(8))) void k_row_tiled(RowPlan plan, const int* __restrict__ loc,
                                                    const T* __restrict__ val, const T* __restrict__ xp,
                                                    T* __restrict__ y, T* __restrict__ yp, T* __restrict__ y0,
                                                    T* __restrict__ ax, T* __restrict__ axp, T* __restrict__ ax0,
@@ -309,13 +311,15 @@ __global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) vo
                                                    Ld lr, const double* __restrict__ rfac, Inst st, int tloc,
                                                    double rho) {
   constexpr bool SPL = kSplit<T, 6>;
-  __shared__ T tile[kTileCols<T> * 64];
-  __shared__ int entL[kTileWaves * 64];
-  __shared__ T entV[kTileWaves * 64];
+  constexpr int WAVES = tile_waves(SHAPE);
+  constexpr int NS = kStageCols<T, SHAPE>;
+  __shared__ T tile[kTileCols<T, SHAPE> * 64];
+  __shared__ int entL[WAVES * 64];
+  __shared__ T entV[WAVES * 64];
   const int t = block_id(plan);
   const int b = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  const TileRec<T> r = tile_record<T>(plan, t);
+  const TileRec<NS> r = tile_record<NS, WAVES>(plan, t);
   const long long it = st.iter[b] + tloc;
   const bool rst = st.applyAt[b] == it;
   const bool live = !st.done[b];
@@ -327,7 +331,7 @@ __global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) vo
     eL = loc[r.e0 + b];
     eV = val[r.e0 + b];
   }
-  T stage[kStageCols<T>];
+  T stage[NS];
   tile_load<SPL>(r, xp, b, live, stage);
   const int i = r.row;
   RowOps<T> ops;
@@ -336,7 +340,7 @@ __global__ __launch_bounds__(kTileWG) __attribute__((amdgpu_waves_per_eu(8))) vo
     entL[wave * 64 + b] = eL;
     entV[wave * 64 + b] = eV;
   }
-  tile_store(r, stage, tile, b);
+  tile_store<WAVES>(r, stage, tile, b);
   if (i < 0) return;
   const T acc = ent ? row_dot_lds_staged<T>(entL + wave * 64, entV + wave * 64, tile, len, b)
                     : row_dot_lds<T>(loc, val, tile, r.e0, r.e1, b);
@@ -677,13 +681,15 @@ struct PlanHost {
   // Tiles for the iterations without checks (B = 64, MapSetup::tile); the
   // check kernels keep `plan`.
   bool tiled = false;
+  int tileShape = 0;  // 1..kTileShapes (tiles.h) when tiled
   RowPlan tplan{};
   TileSet tiles;
   std::vector<int> desc;  // tile_descriptors, kDescInts ints per (tile, wavefront)
   int* dLoc = nullptr;
 };
 
-PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms, int tileCols = 0) {
+// valueBytes: bytes of an iterate element (the tiles' LDS budget in columns).
+PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms, int valueBytes = 0) {
   PlanHost h;
   const int rows = M.rows;
   const int maxLogS = 6 - logB;  // B * S <= 64 lanes (one wavefront per row at most)
@@ -758,11 +764,14 @@ PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms, int tileCols = 0
   h.blocks = std::max(blk, 1);
   h.plan.xcd = ms.xcd ? 1 : 0;
   h.plan.perm = nullptr;
-  if (ms.tile && logB == 6 && tileCols > 0 && build_tiles(M, tileCols, h.tiles)) {
+  const int shape = ms.tile, waves = tile_waves(shape);
+  const int tileCols = shape > 0 && valueBytes > 0 ? tile_bytes(shape) / (64 * valueBytes) : 0;
+  if (shape > 0 && logB == 6 && tileCols > 0 && build_tiles(M, tileCols, h.tiles, waves)) {
     h.tiled = true;
+    h.tileShape = shape;
     h.tplan = h.plan;
     h.tplan.xcd = ms.xcd ? 1 : 0;
-    h.desc = tile_descriptors(M, h.tiles, (tileCols + kTileWaves - 1) / kTileWaves);
+    h.desc = tile_descriptors(M, h.tiles, (tileCols + waves - 1) / waves, waves);
   }
   return h;
 }
@@ -842,8 +851,8 @@ struct Kern {
                               const T*, Ld, Inst, int, double);
   using RowTiledFn = void (*)(RowPlan, const int*, const T*, const T*, T*, T*, T*, T*, T*, T*, const T*, const T*,
                               const double*, const double*, Ld, const double*, Inst, int, double);
-  ColTiledFn colTiled = nullptr;  // B = 64 tiles
-  RowTiledFn rowTiled = nullptr;
+  ColTiledFn colTiled[kTileShapes] = {};  // B = 64 tiles, by shape - 1
+  RowTiledFn rowTiled[kTileShapes] = {};
 };
 
 template <typename T, int LOGB>
@@ -864,8 +873,12 @@ Kern<T> kern_for() {
   k.spmv[1] = k_spmv<T, LOGB, kSplit<T, LOGB>, R2>;
   k.spmvStd = k_spmv<T, LOGB, false>;
   if constexpr (LOGB == 6) {
-    k.colTiled = k_col_tiled<T>;
-    k.rowTiled = k_row_tiled<T>;
+    k.colTiled[0] = k_col_tiled<T, 1>;
+    k.rowTiled[0] = k_row_tiled<T, 1>;
+    k.colTiled[1] = k_col_tiled<T, 2>;
+    k.rowTiled[1] = k_row_tiled<T, 2>;
+    k.colTiled[2] = k_col_tiled<T, 3>;
+    k.rowTiled[2] = k_row_tiled<T, 3>;
   }
   return k;
 }
@@ -916,8 +929,8 @@ class RunImpl final : public DeviceRun {
     }
     rfac_ = upload_dbl(P.rfac);
     cfac_ = upload_dbl(P.cfac);
-    colPlan_ = build_plan(P.AT, logB_, s.colMap, kTileCols<T>);
-    rowPlan_ = build_plan(P.A, logB_, s.rowMap, kTileCols<T>);
+    colPlan_ = build_plan(P.AT, logB_, s.colMap, static_cast<int>(sizeof(T)));
+    rowPlan_ = build_plan(P.A, logB_, s.rowMap, static_cast<int>(sizeof(T)));
     for (PlanHost* h : {&colPlan_, &rowPlan_}) upload_plan(*h);
     // Iterates (per instance) and case data (per instance or shared).
     const size_t nB = static_cast<size_t>(n_) * B_, mB = static_cast<size_t>(m_) * B_;
@@ -1253,8 +1266,8 @@ class RunImpl final : public DeviceRun {
     copy_h2d(st_.done, full.data(), B_);
   }
 
-  void autotune(const Prepared& P, const std::vector<MapSetup>& cand, int reps, int* best, double* ms,
-                double* msHalf) override {
+  void autotune(const Prepared& P, const std::vector<MapSetup>& cand, int reps, int* best, double* msIter,
+                double* msIterHalf) override {
     PDHG_REQUIRE(!cand.empty() && reps >= 1, "autotune: no candidates");
     // Scratch state: zero vectors, every lane live, no restart pending.
     const size_t nB = static_cast<size_t>(n_) * B_, mB = static_cast<size_t>(m_) * B_;
@@ -1272,6 +1285,25 @@ class RunImpl final : public DeviceRun {
     copy_h2d(st_.restartAt, zl.data(), B_);
     copy_h2d(st_.tau, step.data(), B_);
     copy_h2d(st_.sigma, step.data(), B_);
+    // Every candidate is timed inside iterations, column and row kernel in
+    // turn, since a kernel's time depends on what ran before it: the column
+    // kernel's candidates with the run's row mapping, then the row kernel's
+    // with the chosen column mapping.
+    auto iterations = [&] {
+      launch_col(false, 0);
+      launch_row(false, 0);
+      HIP_OK(hipEventRecord(ev0_, stream_));
+      for (int r = 0; r < reps; ++r) {
+        launch_col(false, 0);
+        launch_row(false, 0);
+      }
+      HIP_OK(hipEventRecord(ev1_, stream_));
+      HIP_OK(hipEventSynchronize(ev1_));
+      HIP_OK(hipGetLastError());
+      float el = 0;
+      HIP_OK(hipEventElapsedTime(&el, ev0_, ev1_));
+      return static_cast<double>(el) / reps;
+    };
     int* scratch = buf_.alloc<int>(static_cast<size_t>(std::max(m_, n_)));
     for (int dir = 0; dir < 2; ++dir) {
       PlanHost& cur = dir == 0 ? colPlan_ : rowPlan_;
@@ -1279,7 +1311,7 @@ class RunImpl final : public DeviceRun {
       double bestMs = 0;
       int bestI = -1;
       for (size_t i = 0; i < cand.size(); ++i) {
-        PlanHost h = build_plan(M, logB_, cand[i], kTileCols<T>);
+        PlanHost h = build_plan(M, logB_, cand[i], static_cast<int>(sizeof(T)));
         if (h.tiled) {
           upload_plan(h);  // tiles: their own arrays, kept with the run's buffers
         } else {
@@ -1288,53 +1320,35 @@ class RunImpl final : public DeviceRun {
         }
         const PlanHost keep = cur;
         cur = h;
-        auto one = [&] { dir == 0 ? launch_col(false, 0) : launch_row(false, 0); };
-        one();
-        HIP_OK(hipEventRecord(ev0_, stream_));
-        for (int r = 0; r < reps; ++r) one();
-        HIP_OK(hipEventRecord(ev1_, stream_));
-        HIP_OK(hipEventSynchronize(ev1_));
-        HIP_OK(hipGetLastError());
-        float el = 0;
-        HIP_OK(hipEventElapsedTime(&el, ev0_, ev1_));
+        const double t = iterations();
         cur = keep;
-        const double t = el / reps;
         if (bestI < 0 || t < bestMs) {
           bestMs = t;
           bestI = static_cast<int>(i);
         }
       }
       // Install the winner with its own permutation and partial buffers.
-      PlanHost h = build_plan(M, logB_, cand[bestI], kTileCols<T>);
+      PlanHost h = build_plan(M, logB_, cand[bestI], static_cast<int>(sizeof(T)));
       upload_plan(h);
       cur = h;
       best[dir] = bestI;
-      ms[dir] = bestMs;
-      // The winner again with lanes B/2..B-1 done.
-      msHalf[dir] = bestMs;
-      if (B_ >= 2) {
-        std::vector<int> upper(B_, 0);
-        for (int b = B_ / 2; b < B_; ++b) upper[b] = 1;
-        copy_h2d(st_.done, upper.data(), B_);
-        auto one = [&] { dir == 0 ? launch_col(false, 0) : launch_row(false, 0); };
-        one();
-        HIP_OK(hipEventRecord(ev0_, stream_));
-        for (int r = 0; r < reps; ++r) one();
-        HIP_OK(hipEventRecord(ev1_, stream_));
-        HIP_OK(hipEventSynchronize(ev1_));
-        HIP_OK(hipGetLastError());
-        float el = 0;
-        HIP_OK(hipEventElapsedTime(&el, ev0_, ev1_));
-        msHalf[dir] = el / reps;
-        copy_h2d(st_.done, zeros.data(), B_);
-      }
+    }
+    *msIter = iterations();
+    // The chosen mappings again with lanes B/2..B-1 done.
+    *msIterHalf = *msIter;
+    if (B_ >= 2) {
+      std::vector<int> upper(B_, 0);
+      for (int b = B_ / 2; b < B_; ++b) upper[b] = 1;
+      copy_h2d(st_.done, upper.data(), B_);
+      *msIterHalf = iterations();
+      copy_h2d(st_.done, zeros.data(), B_);
     }
     colPart_ = buf_.alloc<double>(static_cast<size_t>(5) * B_ * colPlan_.blocks);
     rowPart_ = buf_.alloc<double>(static_cast<size_t>(4) * B_ * rowPlan_.blocks);
     launches = 0;
   }
 
-  bool tiled(int side) const override { return side == 0 ? colPlan_.tiled : rowPlan_.tiled; }
+  int tiled(int side) const override { return side == 0 ? colPlan_.tileShape : rowPlan_.tileShape; }
 
   void import_lanes(const LaneState& in) override {
     const int k = in.count;
@@ -1459,7 +1473,8 @@ class RunImpl final : public DeviceRun {
   }
   void launch_col(bool check, int tloc) {
     if (!check && colPlan_.tiled) {
-      hipLaunchKernelGGL(k_.colTiled, dim3(colPlan_.tiles.tiles), dim3(kTileWG), 0, stream_, colPlan_.tplan,
+      hipLaunchKernelGGL(k_.colTiled[colPlan_.tileShape - 1], dim3(colPlan_.tiles.tiles),
+                         dim3(64 * tile_waves(colPlan_.tileShape)), 0, stream_, colPlan_.tplan,
                          colPlan_.dLoc, tVal_, y_, yp_, x_, xp_, x0_, c_, lc_, lo_, up_, lb_, st_, tloc, s_.rho);
       return;
     }
@@ -1470,7 +1485,8 @@ class RunImpl final : public DeviceRun {
   }
   void launch_row(bool check, int tloc) {
     if (!check && rowPlan_.tiled) {
-      hipLaunchKernelGGL(k_.rowTiled, dim3(rowPlan_.tiles.tiles), dim3(kTileWG), 0, stream_, rowPlan_.tplan,
+      hipLaunchKernelGGL(k_.rowTiled[rowPlan_.tileShape - 1], dim3(rowPlan_.tiles.tiles),
+                         dim3(64 * tile_waves(rowPlan_.tileShape)), 0, stream_, rowPlan_.tplan,
                          rowPlan_.dLoc, aVal_, xp_, y_, yp_, y0_, ax_, axp_, ax0_, rl_, ru_, rl64_, ru64_, lr_, rfac_,
                          st_, tloc, s_.rho);
       return;

@@ -11,15 +11,23 @@
 
 namespace pdhgamd {
 
-// A 1024-thread workgroup (16 wavefronts, one row each) per tile, and the
-// LDS it stages the tile's columns in (128 FP64 columns); two such workgroups
-// fill a CU's 32 wavefront slots and 128 of its 160 KB of LDS.
-constexpr int kTileWaves = 16;
+// A workgroup of W wavefronts (one row each) per tile, and the LDS it stages
+// the tile's distinct columns in.  Three shapes, each filling a CU's 32
+// wavefront slots (8 per SIMD) within its 160 KB of LDS:
+//   shape 1: 16 wavefronts, 64 KB of columns (128 FP64), 2 workgroups per CU;
+//   shape 2:  8 wavefronts, 32 KB (64 FP64), 4 workgroups per CU;
+//   shape 3:  4 wavefronts, 16 KB (32 FP64), 8 workgroups per CU.
+// Smaller tiles share fewer columns but let more workgroups, at different
+// points of their load / barrier / sum sequence, share a CU.
+constexpr int kTileShapes = 3;
+constexpr int tile_waves(int shape) { return shape == 3 ? 4 : shape == 2 ? 8 : 16; }
+constexpr int tile_bytes(int shape) { return shape == 3 ? 16 * 1024 : shape == 2 ? 32 * 1024 : 64 * 1024; }
+template <typename T, int SHAPE = 1>
+constexpr int kTileCols = tile_bytes(SHAPE) / (64 * static_cast<int>(sizeof(T)));
+// Shape 1, the largest.
+constexpr int kTileWaves = tile_waves(1);
 constexpr int kTileWG = 64 * kTileWaves;
 constexpr int kTileRows = kTileWaves;
-constexpr int kTileBytes = 64 * 1024;
-template <typename T>
-constexpr int kTileCols = kTileBytes / (64 * static_cast<int>(sizeof(T)));
 
 // Zero entries (column 0, weight 0) after the last nonzero of the CSR arrays
 // the iteration kernels read: whole-chunk scalar loads may run up to a
@@ -42,10 +50,10 @@ struct TileSet {
   int tiles = 0;
 };
 
-// Tiles of at most kTileRows rows and maxCols distinct columns over the rows
+// Tiles of at most maxRows rows and maxCols distinct columns over the rows
 // in locality order (by the column of their middle entry), so that rows of a
 // tile share columns.  False when a row alone has more than maxCols entries.
-inline bool build_tiles(const Csr& M, int maxCols, TileSet& t) {
+inline bool build_tiles(const Csr& M, int maxCols, TileSet& t, int maxRows = kTileRows) {
   const int rows = M.rows;
   for (int i = 0; i < rows; ++i)
     if (M.ptr[i + 1] - M.ptr[i] > maxCols) return false;
@@ -67,7 +75,7 @@ inline bool build_tiles(const Csr& M, int maxCols, TileSet& t) {
     const int i = t.perm[r];
     for (int pass = 0; pass < 2; ++pass) {
       const size_t keep = t.ucols.size();
-      bool fits = inTile < kTileRows;
+      bool fits = inTile < maxRows;
       for (int e = M.ptr[i]; fits && e < M.ptr[i + 1]; ++e) {
         const int c = M.idx[e];
         if (mark[c] == tile) continue;
@@ -100,25 +108,25 @@ inline bool build_tiles(const Csr& M, int maxCols, TileSet& t) {
 // wavefront gets everything it needs before its gathers in one round of
 // scalar loads: [0] its row (-1: none), [1], [2] the row's entry range,
 // [3] the number of tile columns it stages, [4 + k] the k-th of them (tile
-// position wave + k * kTileWaves, the matrix column ucols[tileCol[t] +
+// position wave + k * waves, the matrix column ucols[tileCol[t] +
 // position]).  Records are `desc_stride(stageCols)` ints apart, a multiple of
-// eight so that each starts on a 32-byte boundary.
+// eight so that each starts on a 32-byte boundary.  The tiles hold at most
+// `waves` rows (one per wavefront).
 constexpr int desc_stride(int stageCols) { return (4 + stageCols + 7) / 8 * 8; }
 
-inline std::vector<int> tile_descriptors(const Csr& M, const TileSet& t, int stageCols) {
-  static_assert(kTileRows <= kTileWaves, "one row per wavefront");
+inline std::vector<int> tile_descriptors(const Csr& M, const TileSet& t, int stageCols, int waves = kTileWaves) {
   const int stride = desc_stride(stageCols);
-  std::vector<int> d(static_cast<size_t>(t.tiles) * kTileWaves * stride, 0);
+  std::vector<int> d(static_cast<size_t>(t.tiles) * waves * stride, 0);
   for (int tt = 0; tt < t.tiles; ++tt) {
     const int r0 = t.tileRow[tt], r1 = t.tileRow[tt + 1], u0 = t.tileCol[tt], u1 = t.tileCol[tt + 1];
-    for (int w = 0; w < kTileWaves; ++w) {
-      int* o = &d[(static_cast<size_t>(tt) * kTileWaves + w) * stride];
+    for (int w = 0; w < waves; ++w) {
+      int* o = &d[(static_cast<size_t>(tt) * waves + w) * stride];
       const int j = r0 + w < r1 ? t.perm[r0 + w] : -1;
       o[0] = j;
       o[1] = j >= 0 ? M.ptr[j] : 0;
       o[2] = j >= 0 ? M.ptr[j + 1] : 0;
       int ns = 0;
-      for (int k = 0; k < stageCols && u0 + w + k * kTileWaves < u1; ++k) o[4 + ns++] = t.ucols[u0 + w + k * kTileWaves];
+      for (int k = 0; k < stageCols && u0 + w + k * waves < u1; ++k) o[4 + ns++] = t.ucols[u0 + w + k * waves];
       o[3] = ns;
     }
   }

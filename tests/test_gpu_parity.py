@@ -389,30 +389,41 @@ def test_streamed_family_equals_fixed_batches(gpu):
 
 @pytest.mark.parametrize("prec", [P.FP64, P.FP32])
 def test_tiled_rows_match_one_row_per_wavefront(gpu, prec):
-    """Width 64 with the rows in LDS-staged tiles (row_map 3): each row's
-    entries are summed in the order of the one-row-per-wavefront mapping
-    (row_map 1), so every case ends with identical iterates.  On the
-    log-normal LP the longest rows of A (155 entries) exceed the FP64 staging
-    budget (128 columns), and that side keeps the plain plan."""
+    """Width 64 with the rows in LDS-staged tiles of each shape (row_map 3, 4,
+    5: 16, 8 or 4 rows per tile, 64, 32 or 16 KB of staged columns): each
+    row's entries are summed in the order of the one-row-per-wavefront
+    mapping (row_map 1), so every case ends with identical iterates.  A side
+    whose longest row exceeds the shape's staging budget keeps the plain plan
+    (on the log-normal LP the longest rows of A have 155 entries)."""
+    import scipy.sparse as sp
+
     rng = np.random.default_rng(12)
     small = P.gen_random_lp(11, 120, 240, 4.0, 0.0)
     wide = P.gen_random_lp(2, 3000, 6000, 5.12, 1.0)
-    for lp, tiled_rows in ((small, True), (wide, prec == P.FP32)):
+    vbytes = 8 if prec == P.FP64 else 4
+    for lp in (small, wide):
+        A = sp.csc_matrix((lp.value, lp.index, lp.start), shape=(lp.m, lp.n))
+        longest_col = int(np.diff(A.indptr).max())  # rows of A^T: the column kernel's side
+        longest_row = int(np.diff(A.tocsr().indptr).max())
         costs = lp.c[None, :] + rng.uniform(0.0, 0.2, (64, lp.n))
-        runs = []
-        for row_map in (1, 3):
+        runs = {}
+        for row_map in (1, 3, 4, 5):
             o = P.default_options(tolerance=1e-7, precision=prec, row_map=row_map, xcd_remap=1,
                                   max_iterations=20000, compact=0)
-            runs.append(P.solve_batch(lp, costs, o))
-        (plain, st1, _), (tiled, st3, _) = runs
+            runs[row_map] = P.solve_batch(lp, costs, o)
+        plain, st1, _ = runs[1]
         assert (st1["col_row_map"], st1["row_row_map"]) == (1, 1)
-        assert (st3["col_row_map"], st3["row_row_map"]) == (3, 3 if tiled_rows else 1)
-        for q in range(64):
-            assert tiled[q].status == plain[q].status and tiled[q].iterations == plain[q].iterations, q
-            assert tiled[q].objective == plain[q].objective, q
-            assert np.array_equal(tiled[q].x, plain[q].x) and np.array_equal(tiled[q].y, plain[q].y), q
-        print(f"m={lp.m} n={lp.n}: {sum(r.status == P.OPTIMAL for r in tiled)}/64 optimal, "
-              f"iterations {st3['iterations']}")
+        for row_map, budget in ((3, 64 * 1024), (4, 32 * 1024), (5, 16 * 1024)):
+            tiled, st, _ = runs[row_map]
+            cols = budget // (64 * vbytes)
+            expect = (row_map if longest_col <= cols else 1, row_map if longest_row <= cols else 1)
+            assert (st["col_row_map"], st["row_row_map"]) == expect, (row_map, expect)
+            for q in range(64):
+                assert tiled[q].status == plain[q].status and tiled[q].iterations == plain[q].iterations, q
+                assert tiled[q].objective == plain[q].objective, q
+                assert np.array_equal(tiled[q].x, plain[q].x) and np.array_equal(tiled[q].y, plain[q].y), q
+            print(f"m={lp.m} n={lp.n} row_map {row_map}: tiled sides {expect}, "
+                  f"{sum(r.status == P.OPTIMAL for r in tiled)}/64 optimal, iterations {st['iterations']}")
 
 
 def test_graph_replays_equal_direct_launches(gpu):

@@ -68,7 +68,7 @@ def test_bad_solver_input_is_rejected_before_device_work():
     res = P.ResultStruct()
     assert P.lib().pdhg_solve(C.byref(s), None, C.byref(res), None) == -1
     assert b"null column pointers" in P.lib().pdhg_last_error()
-    for kw, msg in ((dict(precision=7), "unknown precision"), (dict(row_map=4), "row_map"),
+    for kw, msg in ((dict(precision=7), "unknown precision"), (dict(row_map=6), "row_map"),
                     (dict(lanes_per_row=65), "lanes_per_row"), (dict(xcd_remap=2), "xcd_remap"),
                     (dict(tolerance=float("nan")), "tolerance is NaN"), (dict(check_every=1), "check_every"),
                     (dict(max_iterations=0), "max_iterations"), (dict(device=-1), "device ordinal"),

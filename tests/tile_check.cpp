@@ -1,6 +1,6 @@
 // Invariants of the width-64 row tiles (tiles.h), checked on the host:
-// every row in exactly one tile, at most kTileRows rows and maxCols distinct
-// columns per tile, loc[e] naming entry e's column among its tile's columns,
+// every row in exactly one tile, at most `rows` rows (one per wavefront of the
+// tile shape) and maxCols distinct columns per tile, loc[e] naming entry e's column among its tile's columns,
 // the refusal of a matrix with a row longer than maxCols, and the
 // per-(tile, wavefront) records: each wavefront's row and entry range, and
 // every tile column staged by exactly one wavefront at its tile position.
@@ -15,11 +15,11 @@
 using pdhgamd::Csr;
 using pdhgamd::TileSet;
 
-static bool check(const Csr& M, int maxCols, const char* name) {
+static bool check(const Csr& M, int maxCols, int rows, const char* name) {
   TileSet t;
   int maxLen = 0;
   for (int i = 0; i < M.rows; ++i) maxLen = std::max(maxLen, M.ptr[i + 1] - M.ptr[i]);
-  const bool got = pdhgamd::build_tiles(M, maxCols, t);
+  const bool got = pdhgamd::build_tiles(M, maxCols, t, rows);
   if (got != (maxLen <= maxCols)) {
     std::printf("%s: build_tiles returned %d with longest row %d, budget %d\n", name, got, maxLen, maxCols);
     return false;
@@ -38,7 +38,7 @@ static bool check(const Csr& M, int maxCols, const char* name) {
       std::printf("%s: empty tile %d\n", name, k);
       return false;
     }
-    if (r1 - r0 > pdhgamd::kTileRows || u1 - u0 > maxCols || u1 < u0) {
+    if (r1 - r0 > rows || u1 - u0 > maxCols || u1 < u0) {
       std::printf("%s: tile %d has %d rows, %d columns\n", name, k, r1 - r0, u1 - u0);
       return false;
     }
@@ -68,25 +68,26 @@ static bool check(const Csr& M, int maxCols, const char* name) {
       std::printf("%s: row %d in no tile\n", name, i);
       return false;
     }
-  const int stageCols = (maxCols + pdhgamd::kTileWaves - 1) / pdhgamd::kTileWaves;
+  const int waves = rows;
+  const int stageCols = (maxCols + waves - 1) / waves;
   const int stride = pdhgamd::desc_stride(stageCols);
-  const std::vector<int> d = pdhgamd::tile_descriptors(M, t, stageCols);
-  if (d.size() != static_cast<size_t>(t.tiles) * pdhgamd::kTileWaves * stride || stride % 8 != 0) {
+  const std::vector<int> d = pdhgamd::tile_descriptors(M, t, stageCols, waves);
+  if (d.size() != static_cast<size_t>(t.tiles) * waves * stride || stride % 8 != 0) {
     std::printf("%s: descriptor size\n", name);
     return false;
   }
   for (int k = 0; k < t.tiles; ++k) {
     const int r0 = t.tileRow[k], r1 = t.tileRow[k + 1], u0 = t.tileCol[k], u1 = t.tileCol[k + 1];
     std::vector<int> staged(u1 - u0, 0);
-    for (int w = 0; w < pdhgamd::kTileWaves; ++w) {
-      const int* o = &d[(static_cast<size_t>(k) * pdhgamd::kTileWaves + w) * stride];
+    for (int w = 0; w < waves; ++w) {
+      const int* o = &d[(static_cast<size_t>(k) * waves + w) * stride];
       const int j = r0 + w < r1 ? t.perm[r0 + w] : -1;
       if (o[0] != j || (j >= 0 && (o[1] != M.ptr[j] || o[2] != M.ptr[j + 1])) || o[3] < 0 || o[3] > stageCols) {
         std::printf("%s: tile %d wave %d record\n", name, k, w);
         return false;
       }
       for (int q = 0; q < o[3]; ++q) {
-        const int pos = w + q * pdhgamd::kTileWaves;
+        const int pos = w + q * waves;
         if (pos >= u1 - u0 || o[4 + q] != t.ucols[u0 + pos]) {
           std::printf("%s: tile %d wave %d stages a wrong column\n", name, k, w);
           return false;
@@ -123,7 +124,11 @@ int main(int argc, char** argv) {
   int files = 0, random = 0;
   for (int a = 1; a < argc; ++a) {
     const Csr M = load(argv[a]);
-    for (int mc : {pdhgamd::kTileCols<double>, pdhgamd::kTileCols<float>, 64, 45}) ok = check(M, mc, argv[a]) && ok;
+    // Each tile shape's FP64 and FP32 budgets (tiles.h), then small budgets.
+    for (int shape = 1; shape <= pdhgamd::kTileShapes; ++shape)
+      for (int bytes : {8, 4})
+        ok = check(M, pdhgamd::tile_bytes(shape) / (64 * bytes), pdhgamd::tile_waves(shape), argv[a]) && ok;
+    for (int mc : {64, 45}) ok = check(M, mc, pdhgamd::kTileRows, argv[a]) && ok;
     ++files;
   }
   std::mt19937 rng(1);
@@ -141,7 +146,8 @@ int main(int argc, char** argv) {
       M.ptr.push_back(static_cast<int>(M.idx.size()));
     }
     M.val.assign(M.idx.size(), 1.0);
-    ok = check(M, 1 + static_cast<int>(rng() % 30), "random") && ok;
+    ok = check(M, 1 + static_cast<int>(rng() % 30), pdhgamd::tile_waves(1 + trial % pdhgamd::kTileShapes), "random") &&
+         ok;
     ++random;
   }
   std::printf("%s: %d matrices from files, %d random\n", ok ? "ok" : "FAILED", files, random);
