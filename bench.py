@@ -340,8 +340,9 @@ def main():
     ap.add_argument("--budget", type=float, default=330.0,
                     help="wall seconds for the whole run; warmup and timed steps stop when the next would not fit")
     ap.add_argument("--precision", choices=["fp32", "fp64", "fp32auto"], default="fp64")
-    ap.add_argument("--row-map", type=int, default=0, choices=[0, 1, 2, 3],
-                    help="row mapping of the iteration kernels: 0 autotuned, 1 buckets, 2 one mapping, 3 LDS tiles")
+    ap.add_argument("--row-map", type=int, default=0, choices=[0, 1, 2, 3, 4, 5],
+                    help="row mapping of the iteration kernels: 0 autotuned, 1 buckets, 2 one mapping, "
+                         "3, 4, 5 LDS tiles of 16, 8, 4 rows")
     ap.add_argument("--xcd-remap", type=int, default=-1, choices=[-1, 0, 1],
                     help="workgroup placement: -1 autotuned, 0 round-robin over XCDs, 1 contiguous ranges per XCD")
     ap.add_argument("--cases", type=int, default=64, help="family cases solved per step (config C3: 64)")
