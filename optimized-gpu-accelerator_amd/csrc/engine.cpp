@@ -653,6 +653,7 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
       }
       const bool compactNow = move;
       if (compactNow || toFp64) {
+        const auto tMove = Clock::now();
         std::vector<int> fin, live;
         for (int b = 0; b < real; ++b) (done[b] ? fin : live).push_back(b);
         std::vector<int> newly;
@@ -684,12 +685,13 @@ void solve_batch(const Model& base, const std::vector<CaseData>& cases, const pd
             std::fprintf(stderr, "  pdhg batch: iteration %lld, FP32 progress stalled, continuing in FP64\n", it);
         }
         span = next_pow2(static_cast<int>(ids.size()));
-        if (o.progress_seconds > 0)
-          std::fprintf(stderr, "  pdhg batch: iteration %lld, %d live moved into lanes 0..%d of a run of width %d\n",
-                       it, static_cast<int>(ids.size()), static_cast<int>(ids.size()) - 1, B);
         run = make_run(ids, B);
         run->import_lanes(ls);
         run->start(false);
+        if (o.progress_seconds > 0)
+          std::fprintf(stderr,
+                       "  pdhg batch: iteration %lld, %d live moved into lanes 0..%d of a run of width %d (%.1f ms)\n",
+                       it, static_cast<int>(ids.size()), static_cast<int>(ids.size()) - 1, B, 1e3 * since_s(tMove));
         rec.assign(B, CheckRecord{});
         done.assign(B, 0);
         restarts.assign(B, 0);
