@@ -161,7 +161,8 @@ def spmv_rates(lp, reps=200):
             for transpose, key, rows, cols in ((False, "Ax", lp.m, lp.n), (True, "ATy", lp.n, lp.m)):
                 ms = be.time_spmv(transpose, reps)
                 nbytes = lp.nnz * (vs + 4) + (rows + 1) * 4 + (rows + cols) * k * vs
-                r[key] = {"us": ms * 1e3, "GBs": nbytes / (ms * 1e-3) / 1e9, "bytes": nbytes}
+                r[key] = {"us": ms * 1e3, "GBs": nbytes / (ms * 1e-3) / 1e9, "bytes": nbytes,
+                          "layout": be.spmv_layout(transpose)}
             be.close()
             out[f"{name}_B{k}"] = r
     return out
@@ -228,7 +229,8 @@ def spmv_beyond_cache(periods, reps=50):
             ms = be.time_spmv(transpose, reps)
             nbytes = lp.nnz * (vs + 4) + (rows + 1) * 4 + (rows + cols) * vs
             gbs = nbytes / (ms * 1e-3) / 1e9
-            out[f"{name}_{key}"] = {"us": ms * 1e3, "GBs": gbs, "bytes": nbytes, "frac": gbs / PEAK_HBM_GBS}
+            out[f"{name}_{key}"] = {"us": ms * 1e3, "GBs": gbs, "bytes": nbytes, "frac": gbs / PEAK_HBM_GBS,
+                                    "layout": be.spmv_layout(transpose)}
         be.close()
     log("spmv beyond the Infinity Cache: " + ", ".join(
         f"{k} {v['GBs']:.0f} GB/s ({v['frac']:.2f})" for k, v in out.items() if k != "workload"))
@@ -261,7 +263,7 @@ def large_lp_rates(periods, reps=50):
             ms = be.time_spmv(transpose, reps)
             nbytes = lp.nnz * (vs + 4) + (rows + 1) * 4 + (rows + cols) * vs
             r[key] = {"us": ms * 1e3, "GBs": nbytes / (ms * 1e-3) / 1e9, "bytes": nbytes,
-                      "frac": nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+                      "frac": nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, "layout": be.spmv_layout(transpose)}
         be.close()
         out["spmv"][name] = r
         log(f"large-LP spmv {name}: A x {r['Ax']['us']:.1f} us {r['Ax']['GBs']:.0f} GB/s, "

@@ -230,6 +230,12 @@ int pdhg_backend_diff2(pdhg_backend* be, int a, int b, double* out);   /* pdhg.c
 /* Times `reps` back-to-back SpMVs (transpose: 0 = A x, 1 = A^T y) with HIP
  * events on the backend stream; writes the mean kernel time in ms. */
 int pdhg_backend_time_spmv(pdhg_backend* be, int transpose, int reps, double* ms);
+/* The storage the backend's SpMV (transpose: 0 = A x, 1 = A^T y) runs on:
+ * 0 = CSR with the tuned row mapping, 1 = sliced ELLPACK (width 1 only, chosen
+ * when timed faster).  Engine-side; the reference's backend has one form. */
+int pdhg_backend_spmv_layout(pdhg_backend* be, int transpose, int* layout);
+/* Overrides that choice (layout 1 only for a backend of width 1). */
+int pdhg_backend_set_spmv_layout(pdhg_backend* be, int transpose, int layout);
 
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -504,6 +504,46 @@ __global__ __launch_bounds__(kWG) void k_spmv(RowPlan plan, const int* __restric
     if (rows[rr] >= 0 && s.sub == 0) vst<SPL, LOGB>(out, rows[rr], b, accs[rr]);
 }
 
+// Width-1 SpMV over a sliced ELLPACK copy of the matrix (slices of 64 rows,
+// SellDev): a wavefront takes a slice, lane i its row perm[64 s + i].  The
+// slice stores entry k of its 64 rows at soff[s] + 64 k + i (rows shorter
+// than the slice padded with weight 0), so every index and value load is
+// one contiguous run of 64 elements, and neighbouring rows, whose entries
+// share columns, gather together.  Each lane sums its row in entry order
+// over four chains (entry k into chain k mod 4); eight entries' loads are
+// issued before their multiply-adds.
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_spmv_sell(int rows, int slices, const int* __restrict__ soff,
+                                                  const int* __restrict__ sidx, const T* __restrict__ sval,
+                                                  const int* __restrict__ perm, const T* __restrict__ in,
+                                                  T* __restrict__ out) {
+  const int s = blockIdx.x * (kWG / 64) + static_cast<int>(threadIdx.x >> 6);
+  if (s >= slices) return;
+  const int lane = threadIdx.x & 63;
+  const int e0 = __builtin_amdgcn_readfirstlane(soff[s]);
+  const int w = __builtin_amdgcn_readfirstlane((soff[s + 1] - e0) >> 6);
+  const int* __restrict__ ip = sidx + e0 + lane;
+  const T* __restrict__ vp = sval + e0 + lane;
+  T a[4] = {0, 0, 0, 0};
+  int k = 0;
+  for (; k + 8 <= w; k += 8) {
+    int c[8];
+    T v[8], g[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      c[u] = ip[(k + u) * 64];
+      v[u] = vp[(k + u) * 64];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) g[u] = in[c[u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u & 3] += v[u] * g[u];
+  }
+  for (; k < w; ++k) a[k & 3] += vp[k * 64] * in[ip[k * 64]];
+  const int r = s * 64 + lane;
+  if (r < rows) out[perm[r]] = (a[0] + a[1]) + (a[2] + a[3]);
+}
+
 // Vector layout of the op-level kernels: element (i, b) of rows of `ld`
 // lanes at i*ld + b, or, for FP64 with `split`, the run layout of the
 // batched kernels (the low words of row i's ld lanes, then their high words).
@@ -772,6 +812,53 @@ PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms, int valueBytes =
     h.tplan = h.plan;
     h.tplan.xcd = ms.xcd ? 1 : 0;
     h.desc = tile_descriptors(M, h.tiles, (tileCols + waves - 1) / waves, waves);
+  }
+  return h;
+}
+
+// Sliced ELLPACK copy of a CSR matrix for k_spmv_sell: rows ordered by
+// length (longest first) within windows of 512 rows, so that a slice's rows
+// have similar lengths and stay near each other; slice s holds perm[64 s ..
+// 64 s + 63], its width is its longest row, and padding repeats the row's
+// last column (or column 0) with weight 0.
+struct SellHost {
+  std::vector<int> soff, sidx, perm;
+  std::vector<double> sval;
+  int slices = 0;
+};
+
+SellHost build_sell(const Csr& M) {
+  constexpr int kWindow = 512;
+  SellHost h;
+  const int rows = M.rows;
+  h.perm.resize(rows);
+  for (int i = 0; i < rows; ++i) h.perm[i] = i;
+  auto len = [&](int i) { return M.ptr[i + 1] - M.ptr[i]; };
+  for (int w0 = 0; w0 < rows; w0 += kWindow)
+    std::stable_sort(h.perm.begin() + w0, h.perm.begin() + std::min(rows, w0 + kWindow),
+                     [&](int a, int b) { return len(a) > len(b); });
+  h.slices = (rows + 63) / 64;
+  h.soff.assign(h.slices + 1, 0);
+  long long total = 0;
+  for (int s = 0; s < h.slices; ++s) {
+    int w = 0;
+    for (int i = 64 * s; i < std::min(rows, 64 * s + 64); ++i) w = std::max(w, len(h.perm[i]));
+    total += 64LL * w;
+    PDHG_REQUIRE(total < (1LL << 31), "sliced matrix too large for 32-bit offsets");
+    h.soff[s + 1] = static_cast<int>(total);
+  }
+  h.sidx.assign(static_cast<size_t>(total), 0);
+  h.sval.assign(static_cast<size_t>(total), 0.0);
+  for (int s = 0; s < h.slices; ++s) {
+    const int w = (h.soff[s + 1] - h.soff[s]) / 64;
+    for (int lane = 0; lane < 64 && 64 * s + lane < rows; ++lane) {
+      const int r = h.perm[64 * s + lane], e0 = M.ptr[r], n = len(r);
+      for (int k = 0; k < w; ++k) {
+        const size_t at = static_cast<size_t>(h.soff[s]) + 64 * static_cast<size_t>(k) + lane;
+        h.sidx[at] = k < n ? M.idx[e0 + k] : (n > 0 ? M.idx[e0 + n - 1] : 0);
+        h.sval[at] = k < n ? M.val[e0 + k] : 0.0;
+      }
+    }
   }
   return h;
 }
@@ -1576,6 +1663,8 @@ class OpBackendBase {
   virtual void kkt(double* r3, double* c3) = 0;
   virtual void diff2(int a, int b, double* out) = 0;
   virtual double timeSpmv(bool transpose, int reps) = 0;
+  virtual int spmvLayout(bool transpose) const = 0;
+  virtual void setSpmvLayout(bool transpose, int layout) = 0;
 };
 
 // Any k from 1 to 64 instances, as the reference's Backend::init takes
@@ -1624,9 +1713,15 @@ class OpBackend final : public OpBackendBase {
     put(sigma_, one.data(), B_);
     partial_ = buf_.alloc<double>(static_cast<size_t>(kRed) * k * 3);
     // Row mapping of the two SpMVs: the fastest candidate of the batched
-    // engine's autotuner list (LDS tiles aside) for this matrix and width.
-    rowPlan_ = tune_spmv(A, aPtr_, aIdx_, aVal_, vec_[PDHG_V_X], vec_[PDHG_V_AX]);
-    colPlan_ = tune_spmv(AT, tPtr_, tIdx_, tVal_, vec_[PDHG_V_Y], vec_[PDHG_V_ATY]);
+    // engine's autotuner list (LDS tiles aside) for this matrix and width,
+    // and at width 1 the sliced ELLPACK copy when it is faster still.
+    double msA = 0, msAT = 0;
+    rowPlan_ = tune_spmv(A, aPtr_, aIdx_, aVal_, vec_[PDHG_V_X], vec_[PDHG_V_AX], &msA);
+    colPlan_ = tune_spmv(AT, tPtr_, tIdx_, tVal_, vec_[PDHG_V_Y], vec_[PDHG_V_ATY], &msAT);
+    if (B_ == 1) {
+      sellA_ = try_sell(A, vec_[PDHG_V_X], vec_[PDHG_V_AX], msA);
+      sellAT_ = try_sell(AT, vec_[PDHG_V_Y], vec_[PDHG_V_ATY], msAT);
+    }
     HIP_OK(hipStreamSynchronize(stream_));
   }
   ~OpBackend() override {
@@ -1664,6 +1759,11 @@ class OpBackend final : public OpBackendBase {
   void spmv(int in, int out, bool transpose) override {
     check_vec(in);
     check_vec(out);
+    const SellDev& sd = transpose ? sellAT_ : sellA_;
+    if (sd.use) {
+      launch_sell(sd, vec_[in], vec_[out]);
+      return;
+    }
     const PlanHost& p = transpose ? colPlan_ : rowPlan_;
     hipLaunchKernelGGL(kern_.spmv[p.plan.rpw - 1], dim3(p.blocks), dim3(kWG), 0, stream_, p.plan,
                        transpose ? tPtr_ : aPtr_, transpose ? tIdx_ : aIdx_, transpose ? tVal_ : aVal_, vec_[in],
@@ -1740,6 +1840,13 @@ class OpBackend final : public OpBackendBase {
     fetch_sum3(g, r3.data());
     for (int q = 0; q < k_; ++q) out[q] = r3[3 * q];
   }
+  int spmvLayout(bool transpose) const override { return (transpose ? sellAT_ : sellA_).use ? 1 : 0; }
+  void setSpmvLayout(bool transpose, int layout) override {
+    PDHG_REQUIRE(layout == 0 || layout == 1, "spmv layout must be 0 (CSR) or 1 (sliced ELLPACK)");
+    SellDev& d = transpose ? sellAT_ : sellA_;
+    PDHG_REQUIRE(layout == 0 || d.built, "sliced ELLPACK needs a backend of width 1");
+    d.use = layout == 1 && d.rows > 0;
+  }
   double timeSpmv(bool transpose, int reps) override {
     const int in = transpose ? PDHG_V_Y : PDHG_V_X, out = transpose ? PDHG_V_ATY : PDHG_V_AX;
     spmv(in, out, transpose);  // warm
@@ -1800,9 +1907,11 @@ class OpBackend final : public OpBackendBase {
       }
   }
   // Times every candidate mapping of one SpMV (8 launches after a warm one,
-  // HIP events on the backend's stream, zero vectors) and keeps the fastest.
-  // From B = 16 on the XCD remap stays on, as in the engine's tuner.
-  PlanHost tune_spmv(const Csr& M, const int* ptr, const int* idx, const T* val, const T* in, T* out) {
+  // HIP events on the backend's stream, zero vectors) and keeps the fastest
+  // (its mean launch time in *bestMsOut).  From B = 16 on the XCD remap
+  // stays on, as in the engine's tuner.
+  PlanHost tune_spmv(const Csr& M, const int* ptr, const int* idx, const T* val, const T* in, T* out,
+                     double* bestMsOut) {
     std::vector<MapSetup> cand;
     for (bool xcd : {false, true}) {
       if (!xcd && B_ >= 16) continue;
@@ -1842,7 +1951,45 @@ class OpBackend final : public OpBackendBase {
     PlanHost h = build_plan(M, logB_, cand[best]);
     if (!h.perm.empty()) h.dPerm = up(h.perm);
     h.plan.perm = h.dPerm;
+    *bestMsOut = bestMs / 8;
     return h;
+  }
+  // Width 1: the sliced ELLPACK copy of M, timed like the candidates above;
+  // used when faster than the best of them (csrMs per launch).
+  struct SellDev {
+    int* soff = nullptr;
+    int* sidx = nullptr;
+    T* sval = nullptr;
+    int* perm = nullptr;
+    int slices = 0, rows = 0;
+    bool built = false, use = false;
+  };
+  void launch_sell(const SellDev& d, const T* in, T* out) {
+    constexpr int kSlicesPerWG = kWG / 64;
+    hipLaunchKernelGGL(k_spmv_sell<T>, dim3((d.slices + kSlicesPerWG - 1) / kSlicesPerWG), dim3(kWG), 0, stream_,
+                       d.rows, d.slices, d.soff, d.sidx, d.sval, d.perm, in, out);
+    HIP_OK(hipGetLastError());
+  }
+  SellDev try_sell(const Csr& M, const T* in, T* out, double csrMs) {
+    SellDev d;
+    d.built = true;
+    if (M.rows == 0) return d;
+    const SellHost h = build_sell(M);
+    d.soff = up(h.soff);
+    d.sidx = up(h.sidx);
+    d.sval = upT(h.sval);
+    d.perm = up(h.perm);
+    d.slices = h.slices;
+    d.rows = M.rows;
+    launch_sell(d, in, out);
+    HIP_OK(hipEventRecord(e0_, stream_));
+    for (int r = 0; r < 8; ++r) launch_sell(d, in, out);
+    HIP_OK(hipEventRecord(e1_, stream_));
+    HIP_OK(hipEventSynchronize(e1_));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, e0_, e1_));
+    d.use = ms / 8 < csrMs;
+    return d;
   }
   T* upT(const std::vector<double>& v) {
     std::vector<T> t(v.begin(), v.end());
@@ -1861,6 +2008,7 @@ class OpBackend final : public OpBackendBase {
   int *aPtr_, *aIdx_, *tPtr_, *tIdx_;
   T *aVal_, *tVal_;
   PlanHost rowPlan_, colPlan_;
+  SellDev sellA_, sellAT_;
   double *rfac_, *cfac_, *tau_, *sigma_, *partial_;
   T* vec_[PDHG_NV];
   int *iter_, *mask_;
@@ -2033,6 +2181,14 @@ int pdhg_backend_kkt(pdhg_backend* be, double* rows3k, double* cols3k) { PDHG_BE
 int pdhg_backend_diff2(pdhg_backend* be, int a, int b, double* out) { PDHG_BE_CALL(be->impl->diff2(a, b, out)); }
 int pdhg_backend_time_spmv(pdhg_backend* be, int transpose, int reps, double* ms) {
   PDHG_BE_CALL(*ms = be->impl->timeSpmv(transpose != 0, reps < 1 ? 1 : reps));
+}
+
+int pdhg_backend_spmv_layout(pdhg_backend* be, int transpose, int* layout) {
+  PDHG_BE_CALL(*layout = be->impl->spmvLayout(transpose != 0));
+}
+
+int pdhg_backend_set_spmv_layout(pdhg_backend* be, int transpose, int layout) {
+  PDHG_BE_CALL(be->impl->setSpmvLayout(transpose != 0, layout));
 }
 
 int pdhg_kkt_noalloc(int m, int n, int k, int precision, const void* axp, const void* rl, const void* ru,

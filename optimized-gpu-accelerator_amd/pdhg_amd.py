@@ -82,7 +82,8 @@ EXPORTS = [
     "pdhg_backend_create", "pdhg_backend_destroy", "pdhg_backend_set", "pdhg_backend_get",
     "pdhg_backend_spmv", "pdhg_backend_spmvT", "pdhg_backend_set_steps", "pdhg_backend_primal_step",
     "pdhg_backend_dual_step", "pdhg_backend_halpern", "pdhg_backend_restart", "pdhg_backend_kkt",
-    "pdhg_backend_diff2", "pdhg_backend_time_spmv", "pdhg_kkt_noalloc",
+    "pdhg_backend_diff2", "pdhg_backend_time_spmv", "pdhg_backend_spmv_layout",
+    "pdhg_backend_set_spmv_layout", "pdhg_kkt_noalloc",
     "pdhg_safe_dual_bound", "pdhg_certify", "pdhg_certify_batch",
     "pdhg_gen_random_lp", "pdhg_gen_refinery", "pdhg_gen_family", "pdhg_family_costs",
     "pdhg_gen_calibration", "pdhg_read_mps", "pdhg_model_view", "pdhg_model_free",
@@ -140,6 +141,8 @@ def lib() -> C.CDLL:
         L.pdhg_backend_kkt.argtypes = [C.c_void_p, _dp, _dp]
         L.pdhg_backend_diff2.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
         L.pdhg_backend_time_spmv.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.pdhg_backend_spmv_layout.argtypes = [C.c_void_p, C.c_int, _ip]
+        L.pdhg_backend_set_spmv_layout.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.pdhg_kkt_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 4 + [C.c_void_p] + \
             [C.c_void_p] * 5 + [C.c_void_p] * 3 + [C.c_void_p]
         L.pdhg_pod_basis.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
@@ -506,6 +509,17 @@ class Backend:
         ms = C.c_double()
         _check(lib().pdhg_backend_time_spmv(self.h, int(transpose), reps, C.byref(ms)), "time_spmv")
         return ms.value
+
+    def spmv_layout(self, transpose: bool) -> str:
+        """Storage of the SpMV: "csr" (tuned row mapping) or "sell" (sliced ELLPACK, width 1)."""
+        v = C.c_int()
+        _check(lib().pdhg_backend_spmv_layout(self.h, int(transpose), C.byref(v)), "spmv_layout")
+        return "sell" if v.value == 1 else "csr"
+
+    def set_spmv_layout(self, transpose: bool, layout: str) -> None:
+        """Forces the SpMV storage: "csr" or "sell" (width 1 only)."""
+        _check(lib().pdhg_backend_set_spmv_layout(self.h, int(transpose), {"csr": 0, "sell": 1}[layout]),
+               "set_spmv_layout")
 
 
 # ---------------------------------------------------------------------------

@@ -84,6 +84,46 @@ def test_spmv_matches_fp64_product(gpu, prec, k):
     be.close()
 
 
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32])
+def test_sliced_ellpack_spmv_matches_fp64_product(gpu, prec):
+    """Width 1: the backend keeps a sliced ELLPACK copy of A and A^T (slices
+    of 64 rows, rows ordered by length within windows of 512, padded with
+    weight 0) and uses it when it times faster than CSR.  Forced either way,
+    both storages give A x and A^T y within rounding of the FP64 product; the
+    log-normal LP's rows (up to 155 entries) pad slices unevenly, and the
+    matrix of 14 rows checks a slice that is not full.  Other widths refuse
+    the sliced storage."""
+    rng = np.random.default_rng(3)
+    for lp in (P.gen_random_lp(2, 3000, 6000, 5.12, 1.0), P.gen_random_lp(4, 14, 30, 3.0, 0.0)):
+        A, AT, rfac, cfac = scaled_csr(lp)
+        be = P.Backend(A, AT, rfac, cfac, k=1, precision=prec)
+        x = rng.standard_normal(lp.n)
+        y = rng.standard_normal(lp.m)
+        be.set(P.V_X, x)
+        be.set(P.V_Y, y)
+        eps = 4e-14 if prec == P.FP64 else 4e-7 * 64  # about 160 roundings at most per row
+        if prec == P.FP32:
+            x, y = x.astype(np.float32).astype(np.float64), y.astype(np.float32).astype(np.float64)
+            A = sp.csr_matrix((A.data.astype(np.float32).astype(np.float64), A.indices, A.indptr), shape=A.shape)
+            AT = A.T.tocsr()
+        for layout in ("csr", "sell"):
+            for t in (False, True):
+                be.set_spmv_layout(t, layout)
+                assert be.spmv_layout(t) == layout
+            be.spmv(P.V_X, P.V_AX)
+            be.spmvT(P.V_Y, P.V_ATY)
+            ax, aty = be.get(P.V_AX), be.get(P.V_ATY)
+            assert np.all(np.abs(ax - A @ x) <= eps * (abs(A) @ abs(x)) + 1e-300), layout
+            assert np.all(np.abs(aty - AT @ y) <= eps * (abs(AT) @ abs(y)) + 1e-300), layout
+        be.close()
+    lp = P.gen_random_lp(4, 14, 30, 3.0, 0.0)
+    A, AT, rfac, cfac = scaled_csr(lp)
+    be = P.Backend(A, AT, rfac, cfac, k=8, precision=prec)
+    with pytest.raises(RuntimeError, match="width 1"):
+        be.set_spmv_layout(False, "sell")
+    be.close()
+
+
 @pytest.mark.parametrize("prec", [P.FP64, P.FP32, P.FP32_AUTO])
 def test_solve_random_lp_matches_oracle(gpu, prec):
     """FP32 iterates follow their own trajectory, which on this LP depends on
