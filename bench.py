@@ -30,7 +30,7 @@ Cache, plain single-instance GPU PDHG on a sample of the cases, and the
 reference's 16-thread CPU PDHG on a bounded sample of the same workload.
 
 Wall budget: the whole run stays inside --budget seconds (default 330, well
-below the 600 s a driver gives it): warmup and timed steps stop early when
+below a 600 s limit on the run): warmup and timed steps stop early when
 the next step would not fit, and the line reports the steps actually timed.
 After them, when the budget still allows: config C4's 256 uneven LPs in one
 packed run.
