@@ -230,11 +230,14 @@ int pdhg_backend_diff2(pdhg_backend* be, int a, int b, double* out);   /* pdhg.c
 /* Times `reps` back-to-back SpMVs (transpose: 0 = A x, 1 = A^T y) with HIP
  * events on the backend stream; writes the mean kernel time in ms. */
 int pdhg_backend_time_spmv(pdhg_backend* be, int transpose, int reps, double* ms);
-/* The storage the backend's SpMV (transpose: 0 = A x, 1 = A^T y) runs on:
- * 0 = CSR with the tuned row mapping, 1 = sliced ELLPACK (width 1 only, chosen
- * when timed faster).  Engine-side; the reference's backend has one form. */
+/* The storage the backend's SpMV (transpose: 0 = A x, 1 = A^T y) runs on,
+ * the fastest the backend timed: 0 = CSR rows in the tuned mapping, 1 = sliced
+ * ELLPACK (width 1), 2 = CSR rows in LDS-staged tiles (width 33..64).
+ * Engine-side; the reference's backend has one form. */
 int pdhg_backend_spmv_layout(pdhg_backend* be, int transpose, int* layout);
-/* Overrides that choice (layout 1 only for a backend of width 1). */
+/* Forces a storage with its default plan: 0 = one row per wavefront (rows
+ * bucketed by length below width 64), 1 = sliced ELLPACK (width 1 only),
+ * 2 = 8-row LDS tiles (width 33..64 only; -1 if a row exceeds their budget). */
 int pdhg_backend_set_spmv_layout(pdhg_backend* be, int transpose, int layout);
 
 /* No counterpart in the reference: for callers that cannot allocate device

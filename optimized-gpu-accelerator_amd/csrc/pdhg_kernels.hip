@@ -504,6 +504,44 @@ __global__ __launch_bounds__(kWG) void k_spmv(RowPlan plan, const int* __restric
     if (rows[rr] >= 0 && s.sub == 0) vst<SPL, LOGB>(out, rows[rr], b, accs[rr]);
 }
 
+// B = 64 SpMV over LDS-staged row tiles of shape SHAPE (tiles.h): the column
+// and row kernels' tiled product without their update, the same sums as one
+// row per wavefront.
+template <typename T, int SHAPE>
+__global__ __launch_bounds__(64 * tile_waves(SHAPE)) __attribute__((amdgpu_waves_per_eu(8))) void k_spmv_tiled(
+    RowPlan plan, const int* __restrict__ loc, const T* __restrict__ val, const T* __restrict__ in,
+    T* __restrict__ out) {
+  constexpr bool SPL = kSplit<T, 6>;
+  constexpr int WAVES = tile_waves(SHAPE);
+  constexpr int NS = kStageCols<T, SHAPE>;
+  __shared__ T tile[kTileCols<T, SHAPE> * 64];
+  __shared__ int entL[WAVES * 64];
+  __shared__ T entV[WAVES * 64];
+  const int t = block_id(plan);
+  const int b = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const TileRec<NS> r = tile_record<NS, WAVES>(plan, t);
+  const int len = r.e1 - r.e0;
+  const bool ent = len <= 64;
+  int eL = 0;
+  T eV = 0;
+  if (ent && b < len) {
+    eL = loc[r.e0 + b];
+    eV = val[r.e0 + b];
+  }
+  T stage[NS];
+  tile_load<SPL>(r, in, b, true, stage);
+  if (ent && b < len) {
+    entL[wave * 64 + b] = eL;
+    entV[wave * 64 + b] = eV;
+  }
+  tile_store<WAVES>(r, stage, tile, b);
+  if (r.row < 0) return;
+  const T acc = ent ? row_dot_lds_staged<T>(entL + wave * 64, entV + wave * 64, tile, len, b)
+                    : row_dot_lds<T>(loc, val, tile, r.e0, r.e1, b);
+  vst<SPL, 6>(out, r.row, b, acc);
+}
+
 // Width-1 SpMV over a sliced ELLPACK copy of the matrix (slices of 64 rows,
 // SellDev): a wavefront takes a slice, lane i its row perm[64 s + i].  The
 // slice stores entry k of its 64 rows at soff[s] + 64 k + i (rows shorter
@@ -940,6 +978,8 @@ struct Kern {
                               const double*, const double*, Ld, const double*, Inst, int, double);
   ColTiledFn colTiled[kTileShapes] = {};  // B = 64 tiles, by shape - 1
   RowTiledFn rowTiled[kTileShapes] = {};
+  using SpmvTiledFn = void (*)(RowPlan, const int*, const T*, const T*, T*);
+  SpmvTiledFn spmvTiled[kTileShapes] = {};  // op-level SpMV, B = 64 tiles
 };
 
 template <typename T, int LOGB>
@@ -966,6 +1006,9 @@ Kern<T> kern_for() {
     k.rowTiled[1] = k_row_tiled<T, 2>;
     k.colTiled[2] = k_col_tiled<T, 3>;
     k.rowTiled[2] = k_row_tiled<T, 3>;
+    k.spmvTiled[0] = k_spmv_tiled<T, 1>;
+    k.spmvTiled[1] = k_spmv_tiled<T, 2>;
+    k.spmvTiled[2] = k_spmv_tiled<T, 3>;
   }
   return k;
 }
@@ -1676,7 +1719,7 @@ template <typename T>
 class OpBackend final : public OpBackendBase {
  public:
   OpBackend(int m, int n, const Csr& A, const Csr& AT, const double* rfac, const double* cfac, int k, int device)
-      : m_(m), n_(n), k_(k) {
+      : m_(m), n_(n), k_(k), A_(A), AT_(AT) {
     HIP_OK(hipSetDevice(device));
     HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     HIP_OK(hipEventCreate(&e0_));
@@ -1765,9 +1808,18 @@ class OpBackend final : public OpBackendBase {
       return;
     }
     const PlanHost& p = transpose ? colPlan_ : rowPlan_;
-    hipLaunchKernelGGL(kern_.spmv[p.plan.rpw - 1], dim3(p.blocks), dim3(kWG), 0, stream_, p.plan,
-                       transpose ? tPtr_ : aPtr_, transpose ? tIdx_ : aIdx_, transpose ? tVal_ : aVal_, vec_[in],
-                       vec_[out]);
+    launch_csr(p, transpose ? tPtr_ : aPtr_, transpose ? tIdx_ : aIdx_, transpose ? tVal_ : aVal_, vec_[in],
+               vec_[out]);
+  }
+  // One SpMV under plan p: LDS-staged tiles (B = 64, p.tiled) or the row mapping.
+  void launch_csr(const PlanHost& p, const int* ptr, const int* idx, const T* val, const T* in, T* out) {
+    if (p.tiled) {
+      hipLaunchKernelGGL(kern_.spmvTiled[p.tileShape - 1], dim3(p.tiles.tiles), dim3(64 * tile_waves(p.tileShape)), 0,
+                         stream_, p.tplan, p.dLoc, val, in, out);
+    } else {
+      hipLaunchKernelGGL(kern_.spmv[p.plan.rpw - 1], dim3(p.blocks), dim3(kWG), 0, stream_, p.plan, ptr, idx, val, in,
+                         out);
+    }
     HIP_OK(hipGetLastError());
   }
   void setSteps(const double* tau, const double* sigma) override {
@@ -1840,12 +1892,28 @@ class OpBackend final : public OpBackendBase {
     fetch_sum3(g, r3.data());
     for (int q = 0; q < k_; ++q) out[q] = r3[3 * q];
   }
-  int spmvLayout(bool transpose) const override { return (transpose ? sellAT_ : sellA_).use ? 1 : 0; }
+  int spmvLayout(bool transpose) const override {
+    if ((transpose ? sellAT_ : sellA_).use) return 1;
+    return (transpose ? colPlan_ : rowPlan_).tiled ? 2 : 0;
+  }
   void setSpmvLayout(bool transpose, int layout) override {
-    PDHG_REQUIRE(layout == 0 || layout == 1, "spmv layout must be 0 (CSR) or 1 (sliced ELLPACK)");
+    PDHG_REQUIRE(layout >= 0 && layout <= 2, "spmv layout must be 0 (CSR rows), 1 (sliced ELLPACK) or 2 (LDS tiles)");
     SellDev& d = transpose ? sellAT_ : sellA_;
-    PDHG_REQUIRE(layout == 0 || d.built, "sliced ELLPACK needs a backend of width 1");
+    PDHG_REQUIRE(layout != 1 || d.built, "sliced ELLPACK needs a backend of width 1");
+    PDHG_REQUIRE(layout != 2 || B_ == 64, "LDS tiles need a backend of width 33..64");
     d.use = layout == 1 && d.rows > 0;
+    if (layout == 1) return;
+    // The storage's default plan: one row per wavefront (rows bucketed by
+    // length below width 64), or 8-row tiles.
+    const Csr& M = transpose ? AT_ : A_;
+    PlanHost& cur = transpose ? colPlan_ : rowPlan_;
+    MapSetup ms{4, 1, B_ >= 16, false, 1, layout == 2 ? 2 : 0};
+    PlanHost h = build_plan(M, logB_, ms, static_cast<int>(sizeof(T)));
+    PDHG_REQUIRE(layout != 2 || h.tiled, "a row is longer than the tiles' column budget");
+    if (!h.perm.empty()) h.dPerm = up(h.perm);
+    h.plan.perm = h.dPerm;
+    if (h.tiled) up_tiles(h);
+    cur = h;
   }
   double timeSpmv(bool transpose, int reps) override {
     const int in = transpose ? PDHG_V_Y : PDHG_V_X, out = transpose ? PDHG_V_ATY : PDHG_V_AX;
@@ -1917,6 +1985,8 @@ class OpBackend final : public OpBackendBase {
       if (!xcd && B_ >= 16) continue;
       if (B_ == 64) {
         for (int rpw : {1, 2}) cand.push_back(MapSetup{4, 1, xcd, false, rpw});
+        for (int shape = 1; shape <= kTileShapes; ++shape)
+          cand.push_back(MapSetup{4, 1, xcd, false, 1, shape});  // rows in LDS-staged tiles
         continue;
       }
       for (int per : {2, 4, 8, 16})
@@ -1928,13 +1998,14 @@ class OpBackend final : public OpBackendBase {
     double bestMs = 0;
     size_t best = 0;
     for (size_t c = 0; c < cand.size(); ++c) {
-      PlanHost h = build_plan(M, logB_, cand[c]);
-      if (!h.perm.empty()) put(scratch, h.perm.data(), h.perm.size());
-      h.plan.perm = h.perm.empty() ? nullptr : scratch;
-      auto launch = [&] {
-        hipLaunchKernelGGL(kern_.spmv[h.plan.rpw - 1], dim3(h.blocks), dim3(kWG), 0, stream_, h.plan, ptr, idx, val,
-                           in, out);
-      };
+      PlanHost h = build_plan(M, logB_, cand[c], static_cast<int>(sizeof(T)));
+      if (h.tiled) {
+        up_tiles(h);  // their own arrays, kept with the backend's buffers
+      } else {
+        if (!h.perm.empty()) put(scratch, h.perm.data(), h.perm.size());
+        h.plan.perm = h.perm.empty() ? nullptr : scratch;
+      }
+      auto launch = [&] { launch_csr(h, ptr, idx, val, in, out); };
       launch();
       HIP_OK(hipEventRecord(e0_, stream_));
       for (int r = 0; r < 8; ++r) launch();
@@ -1948,11 +2019,18 @@ class OpBackend final : public OpBackendBase {
         best = c;
       }
     }
-    PlanHost h = build_plan(M, logB_, cand[best]);
+    PlanHost h = build_plan(M, logB_, cand[best], static_cast<int>(sizeof(T)));
     if (!h.perm.empty()) h.dPerm = up(h.perm);
     h.plan.perm = h.dPerm;
+    if (h.tiled) up_tiles(h);
     *bestMsOut = bestMs / 8;
     return h;
+  }
+  // Device copies of a tiled plan's records and tile-local columns.
+  void up_tiles(PlanHost& h) {
+    h.tplan.perm = nullptr;
+    h.tplan.desc = up(h.desc);
+    h.dLoc = up(h.tiles.loc);
   }
   // Width 1: the sliced ELLPACK copy of M, timed like the candidates above;
   // used when faster than the best of them (csrMs per launch).
@@ -1999,6 +2077,7 @@ class OpBackend final : public OpBackendBase {
   }
 
   int m_, n_, k_, logB_ = 0, B_ = 1;
+  Csr A_, AT_;  // host copies, for plans built on request (setSpmvLayout)
   bool split_ = false;
   OpLay lay_{1, 0};
   hipStream_t stream_ = nullptr;

@@ -510,15 +510,19 @@ class Backend:
         _check(lib().pdhg_backend_time_spmv(self.h, int(transpose), reps, C.byref(ms)), "time_spmv")
         return ms.value
 
+    LAYOUTS = ("csr", "sell", "tiles")
+
     def spmv_layout(self, transpose: bool) -> str:
-        """Storage of the SpMV: "csr" (tuned row mapping) or "sell" (sliced ELLPACK, width 1)."""
+        """Storage of the SpMV: "csr" (tuned row mapping), "sell" (sliced ELLPACK, width 1) or
+        "tiles" (CSR rows in LDS-staged tiles, width 64)."""
         v = C.c_int()
         _check(lib().pdhg_backend_spmv_layout(self.h, int(transpose), C.byref(v)), "spmv_layout")
-        return "sell" if v.value == 1 else "csr"
+        return self.LAYOUTS[v.value]
 
     def set_spmv_layout(self, transpose: bool, layout: str) -> None:
-        """Forces the SpMV storage: "csr" or "sell" (width 1 only)."""
-        _check(lib().pdhg_backend_set_spmv_layout(self.h, int(transpose), {"csr": 0, "sell": 1}[layout]),
+        """Forces a storage with its default plan: "csr" (one row per wavefront), "sell" (width 1
+        only) or "tiles" (8-row tiles, width 64 only)."""
+        _check(lib().pdhg_backend_set_spmv_layout(self.h, int(transpose), self.LAYOUTS.index(layout)),
                "set_spmv_layout")
 
 

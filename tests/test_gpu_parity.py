@@ -121,7 +121,41 @@ def test_sliced_ellpack_spmv_matches_fp64_product(gpu, prec):
     be = P.Backend(A, AT, rfac, cfac, k=8, precision=prec)
     with pytest.raises(RuntimeError, match="width 1"):
         be.set_spmv_layout(False, "sell")
+    with pytest.raises(RuntimeError, match="width 33..64"):
+        be.set_spmv_layout(False, "tiles")
     be.close()
+
+
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32])
+def test_tiled_spmv_equals_one_row_per_wavefront(gpu, prec):
+    """Width 64: the op-level SpMV on CSR rows in 8-row LDS-staged tiles sums
+    each row in the order of one row per wavefront, so the products are
+    identical, and within rounding of the FP64 product (refinery LP, scaled)."""
+    lp = P.gen_refinery(3)
+    pr = P.prepare(lp)
+    A = sp.csr_matrix((pr["val"], pr["idx"], pr["ptr"]), shape=(lp.m, lp.n))
+    AT = A.T.tocsr()
+    AT.sort_indices()
+    be = P.Backend(A, AT, pr["beta"] / pr["R"], pr["gamma"] / pr["C"], k=64, precision=prec)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((lp.n, 64))
+    y = rng.standard_normal((lp.m, 64))
+    be.set(P.V_X, x)
+    be.set(P.V_Y, y)
+    out = {}
+    for layout in ("csr", "tiles"):
+        for t in (False, True):
+            be.set_spmv_layout(t, layout)
+            assert be.spmv_layout(t) == layout
+        be.spmv(P.V_X, P.V_AX)
+        be.spmvT(P.V_Y, P.V_ATY)
+        out[layout] = (be.get(P.V_AX), be.get(P.V_ATY))
+    be.close()
+    assert np.array_equal(out["csr"][0], out["tiles"][0]) and np.array_equal(out["csr"][1], out["tiles"][1])
+    if prec == P.FP64:
+        ax, aty = out["tiles"]
+        assert np.all(np.abs(ax.reshape(lp.m, 64) - A @ x) <= 1e-14 * (abs(A) @ abs(x)) + 1e-300)
+        assert np.all(np.abs(aty.reshape(lp.n, 64) - AT @ y) <= 1e-14 * (abs(AT) @ abs(y)) + 1e-300)
 
 
 @pytest.mark.parametrize("prec", [P.FP64, P.FP32, P.FP32_AUTO])
