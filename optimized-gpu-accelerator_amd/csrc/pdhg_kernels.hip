@@ -854,11 +854,13 @@ PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms, int valueBytes =
   return h;
 }
 
-// Sliced ELLPACK copy of a CSR matrix for k_spmv_sell: rows ordered by
-// length (longest first) within windows of 512 rows, so that a slice's rows
-// have similar lengths and stay near each other; slice s holds perm[64 s ..
-// 64 s + 63], its width is its longest row, and padding repeats the row's
-// last column (or column 0) with weight 0.
+// Sliced ELLPACK copy of a CSR matrix for k_spmv_sell: slice s holds rows
+// perm[64 s .. 64 s + 63], its width is its longest row, and padding repeats
+// the row's last column (or column 0) with weight 0.  Rows keep their order
+// (neighbours share columns, so they gather together) unless that pads the
+// copy by more than a quarter; then rows are ordered by length (longest
+// first) within windows of 512, so a slice's rows have similar lengths and
+// still come from nearby rows.
 struct SellHost {
   std::vector<int> soff, sidx, perm;
   std::vector<double> sval;
@@ -872,9 +874,19 @@ SellHost build_sell(const Csr& M) {
   h.perm.resize(rows);
   for (int i = 0; i < rows; ++i) h.perm[i] = i;
   auto len = [&](int i) { return M.ptr[i + 1] - M.ptr[i]; };
-  for (int w0 = 0; w0 < rows; w0 += kWindow)
-    std::stable_sort(h.perm.begin() + w0, h.perm.begin() + std::min(rows, w0 + kWindow),
-                     [&](int a, int b) { return len(a) > len(b); });
+  auto padded = [&] {
+    long long total = 0;
+    for (int s0 = 0; s0 < rows; s0 += 64) {
+      int w = 0;
+      for (int i = s0; i < std::min(rows, s0 + 64); ++i) w = std::max(w, len(h.perm[i]));
+      total += 64LL * w;
+    }
+    return total;
+  };
+  if (4 * padded() > 5LL * std::max(1, M.nnz()))
+    for (int w0 = 0; w0 < rows; w0 += kWindow)
+      std::stable_sort(h.perm.begin() + w0, h.perm.begin() + std::min(rows, w0 + kWindow),
+                       [&](int a, int b) { return len(a) > len(b); });
   h.slices = (rows + 63) / 64;
   h.soff.assign(h.slices + 1, 0);
   long long total = 0;

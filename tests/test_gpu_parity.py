@@ -87,12 +87,13 @@ def test_spmv_matches_fp64_product(gpu, prec, k):
 @pytest.mark.parametrize("prec", [P.FP64, P.FP32])
 def test_sliced_ellpack_spmv_matches_fp64_product(gpu, prec):
     """Width 1: the backend keeps a sliced ELLPACK copy of A and A^T (slices
-    of 64 rows, rows ordered by length within windows of 512, padded with
-    weight 0) and uses it when it times faster than CSR.  Forced either way,
-    both storages give A x and A^T y within rounding of the FP64 product; the
-    log-normal LP's rows (up to 155 entries) pad slices unevenly, and the
-    matrix of 14 rows checks a slice that is not full.  Other widths refuse
-    the sliced storage."""
+    of 64 rows padded with weight 0; rows reordered by length within windows
+    of 512 when their own order would pad by more than a quarter) and uses it
+    when it times faster than CSR.  Forced either way, both storages give A x
+    and A^T y within rounding of the FP64 product; the log-normal LP's rows
+    (up to 155 entries) take the reordered copy, and the matrix of 14 rows
+    checks a slice that is not full.  Other widths refuse the sliced
+    storage."""
     rng = np.random.default_rng(3)
     for lp in (P.gen_random_lp(2, 3000, 6000, 5.12, 1.0), P.gen_random_lp(4, 14, 30, 3.0, 0.0)):
         A, AT, rfac, cfac = scaled_csr(lp)
