@@ -854,65 +854,6 @@ PlanHost build_plan(const Csr& M, int logB, const MapSetup& ms, int valueBytes =
   return h;
 }
 
-// Sliced ELLPACK copy of a CSR matrix for k_spmv_sell: slice s holds rows
-// perm[64 s .. 64 s + 63], its width is its longest row, and padding repeats
-// the row's last column (or column 0) with weight 0.  Rows keep their order
-// (neighbours share columns, so they gather together) unless that pads the
-// copy by more than a quarter; then rows are ordered by length (longest
-// first) within windows of 512, so a slice's rows have similar lengths and
-// still come from nearby rows.
-struct SellHost {
-  std::vector<int> soff, sidx, perm;
-  std::vector<double> sval;
-  int slices = 0;
-};
-
-SellHost build_sell(const Csr& M) {
-  constexpr int kWindow = 512;
-  SellHost h;
-  const int rows = M.rows;
-  h.perm.resize(rows);
-  for (int i = 0; i < rows; ++i) h.perm[i] = i;
-  auto len = [&](int i) { return M.ptr[i + 1] - M.ptr[i]; };
-  auto padded = [&] {
-    long long total = 0;
-    for (int s0 = 0; s0 < rows; s0 += 64) {
-      int w = 0;
-      for (int i = s0; i < std::min(rows, s0 + 64); ++i) w = std::max(w, len(h.perm[i]));
-      total += 64LL * w;
-    }
-    return total;
-  };
-  if (4 * padded() > 5LL * std::max(1, M.nnz()))
-    for (int w0 = 0; w0 < rows; w0 += kWindow)
-      std::stable_sort(h.perm.begin() + w0, h.perm.begin() + std::min(rows, w0 + kWindow),
-                       [&](int a, int b) { return len(a) > len(b); });
-  h.slices = (rows + 63) / 64;
-  h.soff.assign(h.slices + 1, 0);
-  long long total = 0;
-  for (int s = 0; s < h.slices; ++s) {
-    int w = 0;
-    for (int i = 64 * s; i < std::min(rows, 64 * s + 64); ++i) w = std::max(w, len(h.perm[i]));
-    total += 64LL * w;
-    PDHG_REQUIRE(total < (1LL << 31), "sliced matrix too large for 32-bit offsets");
-    h.soff[s + 1] = static_cast<int>(total);
-  }
-  h.sidx.assign(static_cast<size_t>(total), 0);
-  h.sval.assign(static_cast<size_t>(total), 0.0);
-  for (int s = 0; s < h.slices; ++s) {
-    const int w = (h.soff[s + 1] - h.soff[s]) / 64;
-    for (int lane = 0; lane < 64 && 64 * s + lane < rows; ++lane) {
-      const int r = h.perm[64 * s + lane], e0 = M.ptr[r], n = len(r);
-      for (int k = 0; k < w; ++k) {
-        const size_t at = static_cast<size_t>(h.soff[s]) + 64 * static_cast<size_t>(k) + lane;
-        h.sidx[at] = k < n ? M.idx[e0 + k] : (n > 0 ? M.idx[e0 + n - 1] : 0);
-        h.sval[at] = k < n ? M.val[e0 + k] : 0.0;
-      }
-    }
-  }
-  return h;
-}
-
 std::mutex gInfoMutex;
 
 }  // namespace

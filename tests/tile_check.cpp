@@ -3,7 +3,8 @@
 // tile shape) and maxCols distinct columns per tile, loc[e] naming entry e's column among its tile's columns,
 // the refusal of a matrix with a row longer than maxCols, and the
 // per-(tile, wavefront) records: each wavefront's row and entry range, and
-// every tile column staged by exactly one wavefront at its tile position.
+// every tile column staged by exactly one wavefront at its tile position;
+// and the sliced ELLPACK copy of the width-1 SpMV.
 // Usage: tile_check [matrix.bin ...]   (int32 rows, cols, nnz, ptr, idx)
 #include <cstdio>
 #include <cstdlib>
@@ -104,6 +105,77 @@ static bool check(const Csr& M, int maxCols, int rows, const char* name) {
   return true;
 }
 
+// The sliced ELLPACK copy (build_sell): perm a permutation of the rows,
+// slice offsets in steps of 64 x the slice's longest row, every row's entries
+// in its lane in CSR order, the rest of its lane padded with weight 0 and a
+// column of the row (or 0), lanes past the last row empty; rows reordered
+// only when their own order pads by more than a quarter.
+static bool check_sell(const Csr& M, const char* name) {
+  const pdhgamd::SellHost h = pdhgamd::build_sell(M);
+  const int rows = M.rows;
+  auto len = [&](int i) { return M.ptr[i + 1] - M.ptr[i]; };
+  if (h.slices != (rows + 63) / 64 || static_cast<int>(h.soff.size()) != h.slices + 1 || h.soff[0] != 0 ||
+      static_cast<int>(h.perm.size()) != rows || h.sidx.size() != h.sval.size() ||
+      static_cast<long long>(h.sidx.size()) != h.soff[h.slices]) {
+    std::printf("%s: sliced copy sizes\n", name);
+    return false;
+  }
+  std::vector<int> seen(rows, 0);
+  bool identity = true;
+  long long natural = 0;
+  for (int s0 = 0; s0 < rows; s0 += 64) {
+    int w = 0;
+    for (int i = s0; i < std::min(rows, s0 + 64); ++i) w = std::max(w, len(i));
+    natural += 64LL * w;
+  }
+  for (int s = 0; s < h.slices; ++s) {
+    const int off = h.soff[s], w = (h.soff[s + 1] - off) / 64;
+    if ((h.soff[s + 1] - off) % 64 != 0) {
+      std::printf("%s: slice %d size\n", name, s);
+      return false;
+    }
+    int longest = 0;
+    for (int lane = 0; lane < 64; ++lane) {
+      const int pos = 64 * s + lane;
+      const int r = pos < rows ? h.perm[pos] : -1;
+      if (pos < rows && (r < 0 || r >= rows || seen[r]++)) {
+        std::printf("%s: perm is not a permutation\n", name);
+        return false;
+      }
+      identity = identity && (pos >= rows || r == pos);
+      const int n = r >= 0 ? len(r) : 0;
+      longest = std::max(longest, n);
+      for (int k = 0; k < w; ++k) {
+        const size_t at = static_cast<size_t>(off) + 64 * static_cast<size_t>(k) + lane;
+        const bool entry = k < n;
+        const int wantIdx = entry ? M.idx[M.ptr[r] + k] : (n > 0 ? M.idx[M.ptr[r] + n - 1] : 0);
+        const double wantVal = entry ? M.val[M.ptr[r] + k] : 0.0;
+        if (h.sidx[at] != wantIdx || h.sval[at] != wantVal) {
+          std::printf("%s: slice %d lane %d entry %d\n", name, s, lane, k);
+          return false;
+        }
+      }
+    }
+    if (longest != w) {
+      std::printf("%s: slice %d width %d, longest row %d\n", name, s, w, longest);
+      return false;
+    }
+  }
+  // The order expected: the rows' own, or by length (longest first, stable)
+  // within windows of 512 when their own order pads by more than a quarter.
+  std::vector<int> want(rows);
+  for (int i = 0; i < rows; ++i) want[i] = i;
+  if (4 * natural > 5LL * std::max(1, M.nnz()))
+    for (int w0 = 0; w0 < rows; w0 += 512)
+      std::stable_sort(want.begin() + w0, want.begin() + std::min(rows, w0 + 512),
+                       [&](int x, int y) { return len(x) > len(y); });
+  if (want != h.perm) {
+    std::printf("%s: row order of the sliced copy (identity %d)\n", name, identity ? 1 : 0);
+    return false;
+  }
+  return true;
+}
+
 static Csr load(const char* f) {
   std::ifstream in(f, std::ios::binary);
   Csr M;
@@ -115,7 +187,8 @@ static Csr load(const char* f) {
   M.idx.resize(nnz);
   in.read(reinterpret_cast<char*>(M.ptr.data()), 4 * (M.rows + 1));
   in.read(reinterpret_cast<char*>(M.idx.data()), 4 * static_cast<size_t>(nnz));
-  M.val.assign(nnz, 1.0);
+  M.val.resize(nnz);
+  for (int e = 0; e < nnz; ++e) M.val[e] = 1.0 + e;  // distinct, so misplaced values show
   return M;
 }
 
@@ -129,6 +202,7 @@ int main(int argc, char** argv) {
       for (int bytes : {8, 4})
         ok = check(M, pdhgamd::tile_bytes(shape) / (64 * bytes), pdhgamd::tile_waves(shape), argv[a]) && ok;
     for (int mc : {64, 45}) ok = check(M, mc, pdhgamd::kTileRows, argv[a]) && ok;
+    ok = check_sell(M, argv[a]) && ok;
     ++files;
   }
   std::mt19937 rng(1);
@@ -145,10 +219,27 @@ int main(int argc, char** argv) {
       M.idx.insert(M.idx.end(), c.begin(), c.begin() + len);
       M.ptr.push_back(static_cast<int>(M.idx.size()));
     }
-    M.val.assign(M.idx.size(), 1.0);
+    M.val.resize(M.idx.size());
+    for (size_t e = 0; e < M.val.size(); ++e) M.val[e] = 1.0 + static_cast<double>(e);
     ok = check(M, 1 + static_cast<int>(rng() % 30), pdhgamd::tile_waves(1 + trial % pdhgamd::kTileShapes), "random") &&
          ok;
+    ok = check_sell(M, "random") && ok;
     ++random;
+  }
+  {  // uneven rows (mostly 1 entry, every 37th row 200): the sliced copy reorders them
+    Csr M;
+    M.rows = 2000;
+    M.cols = 500;
+    for (int i = 0; i < M.rows; ++i) {
+      const int len = i % 37 == 0 ? 200 : 1;
+      for (int k = 0; k < len; ++k) M.idx.push_back((i + 3 * k) % M.cols);
+      std::sort(M.idx.end() - len, M.idx.end());
+      M.idx.erase(std::unique(M.idx.end() - len, M.idx.end()), M.idx.end());
+      M.ptr.push_back(static_cast<int>(M.idx.size()));
+    }
+    M.val.resize(M.idx.size());
+    for (size_t e = 0; e < M.val.size(); ++e) M.val[e] = 1.0 + static_cast<double>(e);
+    ok = check_sell(M, "uneven") && ok;
   }
   std::printf("%s: %d matrices from files, %d random\n", ok ? "ok" : "FAILED", files, random);
   return ok ? 0 : 1;
